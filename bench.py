@@ -1,0 +1,230 @@
+"""Benchmark: 30-qubit FP64 random circuit on one MI355X.
+
+One step = reset to |0...0> and execute the whole 40-layer circuit (1800
+gates: per layer one gate per qubit from {H,X,Y,Z,RX,RY,RZ,P,U} and 15 CX/CZ
+on a random perfect matching) on a 16 GiB state resident in HBM.
+
+Prints ONE JSON line:
+  value      gates/s over the timed steps (whole job, one GPU)
+  roofline   the dominant kernel class: algorithmic HBM bytes / device time,
+             measured with HIP events on the library stream over the timed
+             region, against the 8 TB/s HBM peak
+  cpu_baseline  the oracle (C restatement of the reference's rayon path,
+             OpenMP) timed on this host on a bounded sample of the workload
+  per_stride plain-kernel GB/s per target qubit for the gate classes
+  replica    max |amplitude error| of the GPU run against the oracle on the
+             26-qubit replica of the circuit (first few layers)
+"""
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK = 8.0e12
+
+
+def parse():
+    p = argparse.ArgumentParser()
+    p.add_argument("--gpus", type=int, default=1)
+    p.add_argument("--steps", type=int, default=3)
+    p.add_argument("--warmup", type=int, default=1)
+    p.add_argument("--qubits", type=int, default=30)
+    p.add_argument("--layers", type=int, default=40)
+    p.add_argument("--seed", type=int, default=2)
+    p.add_argument("--mode", choices=["auto", "plain", "strict"], default="auto")
+    p.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    p.add_argument("--no-replica", action="store_true", help="skip the 26q parity replica")
+    p.add_argument("--no-strides", action="store_true", help="skip the per-stride sweep")
+    p.add_argument("--cpu-threads", type=int, default=0)
+    p.add_argument("--replica-layers", type=int, default=4)
+    return p.parse_args()
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def per_stride(spz, N, n):
+    """GB/s of the plain kernels per target qubit (one launch each, event-timed)."""
+    from spinoza_amd.workloads import gate_bytes
+    s = spz.State(n)
+    classes = {
+        "H": lambda t: spz.apply(spz.Gate.H, s, t),
+        "RZ": lambda t: spz.apply(spz.Gate.RZ(0.3), s, t),
+        "P": lambda t: spz.apply(spz.Gate.P(0.3), s, t),
+        "CX(c=t+1 mod n)": lambda t: spz.c_apply(spz.Gate.X, s, (t + 1) % n, t),
+    }
+    byts = {
+        "H": gate_bytes({"kind": N.H}, n),
+        "RZ": gate_bytes({"kind": N.RZ}, n),
+        "P": gate_bytes({"kind": N.P}, n),
+        "CX(c=t+1 mod n)": gate_bytes({"kind": N.X, "ones": 1}, n),
+    }
+    out = {}
+    for name, fn in classes.items():
+        row = []
+        for t in range(n):
+            fn(t)  # warm
+            N.check(N.lib().spz_sync())
+            N.check(N.lib().spz_profile_reset())
+            reps = 3
+            for _ in range(reps):
+                fn(t)
+            tot = 0.0
+            import ctypes
+            for cls in (N.K_PAIR, N.K_DIAG):
+                ms = ctypes.c_double()
+                N.check(N.lib().spz_profile_get(cls, None, ctypes.byref(ms), None))
+                tot += ms.value
+            row.append(round(byts[name] * reps / (tot * 1e-3) / 1e9, 1))
+        out[name] = row
+    del s
+    return out
+
+
+def cpu_baseline(ops, n, threads, budget_s=20.0):
+    """Time the oracle (OpenMP over the reference's pair loops) on the first
+    gates of the same workload until about `budget_s` of work."""
+    import numpy as np
+    from oracle import pyoracle
+    pyoracle.set_threads(threads)
+    st = pyoracle.OracleState(n)
+    done = 0
+    t0 = time.perf_counter()
+    for o in ops:
+        pyoracle.execute(st, [o])
+        done += 1
+        if time.perf_counter() - t0 > budget_s:
+            break
+    dt = time.perf_counter() - t0
+    del st
+    return done, dt
+
+
+def main():
+    a = parse()
+    if a.gpus != 1:
+        print(json.dumps({"error": "this build runs on exactly one GPU; --gpus must be 1 "
+                          "(multi-GPU sharding is a design note, not built)"}))
+        sys.exit(2)
+    import ctypes
+    import spinoza_amd as spz
+    from spinoza_amd import _native as N
+    from spinoza_amd.executor import encode_ops, run_ops
+    from spinoza_amd.workloads import gate_bytes, random_circuit
+
+    n, L = a.qubits, a.layers
+    mode = {"auto": N.EXEC_AUTO, "plain": N.EXEC_PLAIN, "strict": N.EXEC_STRICT}[a.mode]
+    ops = random_circuit(n, L, a.seed)
+    enc = encode_ops(ops)
+    alg_bytes_circuit = sum(gate_bytes(o, n) for o in ops)
+    log(f"workload: {n}q, {L} layers, {len(ops)} gates, mode={a.mode}")
+
+    s = spz.State(n)
+    for w in range(a.warmup):
+        s.set_basis(0)
+        run_ops(s, ops, mode=mode, encoded=enc)
+        N.check(N.lib().spz_sync())
+        log(f"warmup {w} done")
+
+    N.check(N.lib().spz_profile_enable(1))
+    N.check(N.lib().spz_profile_reset())
+    N.check(N.lib().spz_sync())
+    t0 = time.perf_counter()
+    for k in range(a.steps):
+        s.set_basis(0)
+        run_ops(s, ops, mode=mode, encoded=enc)
+    N.check(N.lib().spz_sync())
+    t1 = time.perf_counter()
+    log(f"timed {a.steps} steps in {t1 - t0:.3f}s")
+
+    prof = {}
+    for cls, name in enumerate(N.K_NAMES):
+        cnt, ms, by = ctypes.c_uint64(), ctypes.c_double(), ctypes.c_double()
+        N.check(N.lib().spz_profile_get(cls, ctypes.byref(cnt), ctypes.byref(ms), ctypes.byref(by)))
+        prof[name] = (cnt.value, ms.value, by.value)
+    N.check(N.lib().spz_profile_enable(0))
+    dom = max((k for k in prof if k != "other"), key=lambda k: prof[k][1])
+    cnt, ms, by = prof[dom]
+    achieved = by / (ms * 1e-3) if ms > 0 else 0.0
+
+    ms_step = (t1 - t0) * 1e3 / a.steps
+    value = len(ops) * a.steps / (t1 - t0)
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", f"traffic_{dom}_{n}q.json")
+    if os.path.exists(tpath):
+        with open(tpath) as f:
+            traffic = json.load(f).get("hbm_bytes_per_launch")
+
+    result = {
+        "metric": "30q f64 gates/s + % HBM roofline; 24q adjoint grads/s vs param-shift",
+        "value": round(value, 2),
+        "unit": "gates/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": round(ms_step, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": f"{n}-qubit random circuit, FP64 SoA state, {L} layers",
+                   "qubits": n, "layers": L, "gates_per_step": len(ops), "seed": a.seed,
+                   "executor": a.mode, "state_bytes": 16 * (1 << n)},
+        "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(achieved / 1e9, 1),
+                     "peak": HBM_PEAK / 1e9, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK, 4), "traffic": traffic,
+                     "launches": cnt, "avg_launch_ms": round(ms / max(cnt, 1), 4),
+                     "alg_bytes_per_launch": by / max(cnt, 1)},
+        "kernel_time_ms": {k: round(v[1], 2) for k, v in prof.items() if v[0]},
+        "alg_gb_per_step": round(alg_bytes_circuit / 1e9, 2),
+        "effective_gb_s": round(alg_bytes_circuit * a.steps / (t1 - t0) / 1e9, 1),
+    }
+    del s
+
+    if not a.no_strides:
+        log("per-stride sweep")
+        result["per_stride_gb_s"] = per_stride(spz, N, n)
+
+    if not a.no_replica:
+        from oracle import pyoracle
+        import numpy as np
+        rn = 26
+        rops = random_circuit(rn, a.replica_layers, a.seed)
+        threads = a.cpu_threads or min(16, os.cpu_count() or 1)
+        pyoracle.set_threads(threads)
+        log(f"26q replica: {len(rops)} gates on the oracle ({threads} threads)")
+        ref = pyoracle.OracleState(rn)
+        pyoracle.execute(ref, rops)
+        g = spz.State(rn)
+        run_ops(g, rops, mode=mode)
+        re, im = g.download()
+        err = float(max(np.max(np.abs(re - ref.re)), np.max(np.abs(im - ref.im))))
+        result["replica"] = {"qubits": rn, "layers": a.replica_layers, "gates": len(rops),
+                             "max_abs_err": err,
+                             "bit_identical": bool(np.array_equal(re, ref.re) and
+                                                   np.array_equal(im, ref.im))}
+        del g, ref
+
+    if not a.no_cpu:
+        threads = a.cpu_threads or min(16, os.cpu_count() or 1)
+        log(f"cpu baseline: oracle, {threads} threads, first gates of the {n}q workload")
+        done, dt = cpu_baseline(ops, n, threads)
+        result["cpu_baseline"] = {
+            "value": round(done / dt, 4), "unit": "gates/s", "cores": threads,
+            "kind": "port",
+            "sample": f"first {done} gates of the same {n}q circuit ({dt:.1f} s), "
+                      f"OpenMP restatement of the reference's rayon pair loops"}
+        result["speedup_vs_cpu"] = round(value / (done / dt), 1)
+
+    print(json.dumps(result), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,303 @@
+"""GPU parity: the gfx950 kernels against the CPU oracle, through the C ABI.
+
+Bar: max |amplitude difference| <= 1e-12 (the tolerance the target states for
+FP64); the plain path is also expected to be bit-identical to the oracle
+(same operations in the same order, no FMA contraction), which is asserted
+separately so a loss of bit-exactness is visible even inside the tolerance.
+"""
+import math
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-12
+KINDS = ["H", "X", "Y", "Z", "P", "RX", "RY", "RZ", "U"]
+
+
+@pytest.fixture(scope="module")
+def spz():
+    import spinoza_amd
+    return spinoza_amd
+
+
+def to_gpu(spz, os_, precision="f64"):
+    s = spz.State(os_.n, precision)
+    s.upload(os_.re, os_.im)
+    return s
+
+
+def assert_match(prod, ref, exact=True, tol=TOL, what=""):
+    re, im = prod.download()
+    err = max(np.max(np.abs(re - ref.re)), np.max(np.abs(im - ref.im)))
+    assert err <= tol, f"{what}: max abs error {err}"
+    if exact:
+        assert np.array_equal(re, ref.re) and np.array_equal(im, ref.im), \
+            f"{what}: not bit-identical (max err {err})"
+
+
+def gate(spz, kind, th=0.0, ph=0.0, la=0.0):
+    G = spz.Gate
+    return {"H": G.H, "X": G.X, "Y": G.Y, "Z": G.Z, "P": G.P(th), "RX": G.RX(th),
+            "RY": G.RY(th), "RZ": G.RZ(th), "U": G.U(th, ph, la)}[kind]
+
+
+@pytest.mark.parametrize("n", [1, 2, 9, 16])
+def test_apply_every_kind_every_target(spz, orc, n):
+    print(f"case apply n={n}", flush=True)
+    rng = np.random.default_rng(n)
+    base = orc.random_state(n, n)
+    for kind in KINDS:
+        for t in range(n):
+            th, ph, la = rng.uniform(0, 2 * np.pi, 3)
+            ref = base.copy()
+            ref.apply(kind, t, th, ph, la)
+            s = to_gpu(spz, base)
+            spz.apply(gate(spz, kind, th, ph, la), s, t)
+            assert_match(s, ref, what=f"{kind} t={t}")
+
+
+@pytest.mark.parametrize("n", [2, 7, 12])
+def test_c_apply_every_pair(spz, orc, n):
+    print(f"case c_apply n={n}", flush=True)
+    rng = np.random.default_rng(100 + n)
+    base = orc.random_state(200 + n, n)
+    for kind in KINDS:
+        for c in range(n):
+            for t in range(n):
+                if c == t:
+                    continue
+                th, ph, la = rng.uniform(0, 2 * np.pi, 3)
+                ref = base.copy()
+                ref.c_apply(kind, c, t, th, ph, la)
+                s = to_gpu(spz, base)
+                spz.c_apply(gate(spz, kind, th, ph, la), s, c, t)
+                assert_match(s, ref, what=f"C{kind} c={c} t={t}")
+
+
+def test_multi_control_masks(spz, orc):
+    print("case mc_apply", flush=True)
+    n = 9
+    rng = np.random.default_rng(7)
+    base = orc.random_state(9, n)
+    for trial in range(60):
+        kind = KINDS[trial % len(KINDS)]
+        qs = rng.permutation(n)
+        t = int(qs[0])
+        ones = sum(1 << int(q) for q in qs[1:1 + rng.integers(1, 4)])
+        zeros = sum(1 << int(q) for q in qs[4:4 + rng.integers(0, 3)])
+        th, ph, la = rng.uniform(0, 2 * np.pi, 3)
+        ref = base.copy()
+        ref.mc_apply(kind, ones, zeros, t, th, ph, la)
+        s = to_gpu(spz, base)
+        spz.mc_apply_masked(gate(spz, kind, th, ph, la), s, ones, zeros, t)
+        assert_match(s, ref, what=f"mc {kind} t={t} ones={ones:b} zeros={zeros:b}")
+    # cc_apply (Toffoli) against the oracle
+    for c0, c1, t in [(0, 1, 2), (2, 0, 1), (5, 8, 0), (3, 7, 6)]:
+        ref = base.copy()
+        ref.mc_apply("X", (1 << c0) | (1 << c1), 0, t)
+        s = to_gpu(spz, base)
+        spz.cc_apply(spz.Gate.X, s, c0, c1, t)
+        assert_match(s, ref, what="ccx")
+
+
+def test_swap_every_pair(spz, orc):
+    print("case swap", flush=True)
+    n = 8
+    base = orc.random_state(3, n)
+    for a in range(n):
+        for b in range(n):
+            ref = base.copy()
+            ref.swap(a, b)
+            s = to_gpu(spz, base)
+            spz.apply(spz.Gate.SWAP(a, b), s, 0)
+            assert_match(s, ref, what=f"swap {a} {b}")
+
+
+def test_dense_blocks(spz, orc):
+    print("case transform_u", flush=True)
+    from spinoza_amd import Unitary
+    n = 9
+    rng = np.random.default_rng(3)
+    base = orc.random_state(33, n)
+    for m in (1, 2, 3, 4, 5):
+        a = rng.normal(size=(1 << m, 1 << m)) + 1j * rng.normal(size=(1 << m, 1 << m))
+        q, _ = np.linalg.qr(a)
+        u = Unitary.from_matrix(q)
+        for t in range(0, n - m + 1, 2):
+            ref = base.copy()
+            ref.transform_u(q, t)
+            s = to_gpu(spz, base)
+            spz.transform_u(s, u, t)
+            assert_match(s, ref, what=f"dense m={m} t={t}")
+            c = (t + m) % n
+            if not (t <= c < t + m):
+                ref = base.copy()
+                ref.transform_u(q, t, control=c)
+                s = to_gpu(spz, base)
+                spz.c_transform_u(s, u, c, t)
+                assert_match(s, ref, what=f"cdense m={m} t={t} c={c}")
+
+
+def test_reference_kats(spz, orc, kats):
+    """The reference's own known-answer tests, through the product."""
+    from spinoza_amd import _native as N
+    from spinoza_amd.executor import run_ops
+    from tests.kat_ops import expected_vectors, kat_op_dicts
+    for case in kats["gate_cases"]:
+        print(f"case kat {case['name']}", flush=True)
+        for mode in (N.EXEC_PLAIN, N.EXEC_AUTO):
+            s = spz.State(case["n"])
+            run_ops(s, kat_op_dicts(case), mode=mode)
+            idx, re, im = expected_vectors(case)
+            got_re, got_im = s.download()
+            np.testing.assert_allclose(got_re[idx], re, atol=case["tol"], rtol=0, err_msg=case["name"])
+            np.testing.assert_allclose(got_im[idx], im, atol=case["tol"], rtol=0, err_msg=case["name"])
+
+
+def test_measure_known_state_kat(spz, kats):
+    print("case measure kat", flush=True)
+    kc = kats["measure_known_state"]
+    v = kc["interleaved_re_im"]
+    s = spz.State.from_arrays(np.array(v[0::2]), np.array(v[1::2]))
+    for step in kc["steps"]:
+        out = spz.measure_qubit(s, step["target"], step["reset"], step["forced"])
+        assert out == step["forced"]
+        re, im = s.download()
+        idx = [a[0] for a in step["amps"]]
+        np.testing.assert_allclose(re[idx], [a[1] for a in step["amps"]], atol=kc["tol"])
+        np.testing.assert_allclose(im[idx], [a[2] for a in step["amps"]], atol=kc["tol"])
+
+
+@pytest.mark.parametrize("n", [3, 13, 20])
+def test_reductions_bit_exact(spz, orc, n):
+    print(f"case reductions n={n}", flush=True)
+    from spinoza_amd import core
+    base = orc.random_state(50 + n, n)
+    s = to_gpu(spz, base)
+    assert s.norm() == base.norm()
+    for t in sorted({0, 1, n // 2, n - 1}):
+        assert core.prob0(s, t) == base.prob0(t)
+        assert spz.qubit_expectation_value(s, t) == base.qubit_expectation_value(t)
+        for obs in "xyz":
+            assert spz.xyz_expectation_value(obs, s, [t]) == base.xyz_expectation_value(obs, [t])
+    assert spz.expect_zz(s, 0, n - 1) == base.expect_zz(0, n - 1)
+    assert spz.ising_energy(s, 1.0) == base.ising_energy(1.0)
+    with pytest.raises(ValueError):
+        spz.xyz_expectation_value("a", s, [0])
+
+
+@pytest.mark.parametrize("n", [4, 14])
+def test_measurement_outcomes_bit_exact(spz, orc, n):
+    print(f"case measure n={n}", flush=True)
+    base = orc.random_state(60 + n, n)
+    us = orc.uniforms(61 + n, 3 * n)
+    ref = base.copy()
+    s = to_gpu(spz, base)
+    k = 0
+    for rnd in range(3):
+        for t in range(n):
+            u = float(us[k]); k += 1
+            reset = bool((t + rnd) & 1)
+            v_ref, _ = ref.measure_qubit(t, reset=reset, forced=-1, u=u)
+            v = spz.measure_qubit(s, t, reset, u=u)
+            assert v == v_ref
+            assert_match(s, ref, what=f"collapse t={t}")
+
+
+def test_qft20_config(spz, orc):
+    """The 20-qubit QFT workload: Gaussian input, all 2^20 amplitudes within
+    1e-12 of the oracle and 4096 samples bit-identical."""
+    print("case qft20", flush=True)
+    from spinoza_amd import _native as N
+    from spinoza_amd.executor import run_ops
+    from spinoza_amd.workloads import qft
+    from oracle import textbook as tb
+    n = 20
+    base = orc.gaussian_state(1, n)
+    ref = base.copy()
+    orc.execute(ref, qft(n))
+    for mode in (N.EXEC_PLAIN, N.EXEC_AUTO):
+        s = to_gpu(spz, base)
+        run_ops(s, qft(n), mode=mode)
+        assert_match(s, ref, exact=(mode == N.EXEC_PLAIN), what=f"qft20 mode={mode}")
+    # closed form: the QFT is the DFT
+    dft = tb.qft_matrix_dft(base.re + 1j * base.im)
+    assert np.max(np.abs((ref.re + 1j * ref.im) - dft)) < 1e-12
+    u = orc.uniforms(2, 4096)
+    assert np.array_equal(spz.sample(s, u), ref.sample(u))
+
+
+def test_random_circuit_vs_oracle(spz, orc):
+    print("case random circuit 18q", flush=True)
+    from spinoza_amd import _native as N
+    from spinoza_amd.executor import run_ops
+    from spinoza_amd.workloads import random_circuit
+    n = 18
+    ops = random_circuit(n, 6, 11)
+    ref = orc.OracleState(n)
+    orc.execute(ref, ops)
+    for mode in (N.EXEC_PLAIN, N.EXEC_STRICT, N.EXEC_AUTO):
+        s = spz.State(n)
+        run_ops(s, ops, mode=mode)
+        assert_match(s, ref, exact=(mode != N.EXEC_AUTO), what=f"random mode={mode}")
+
+
+def test_error_paths(spz):
+    print("case errors", flush=True)
+    s = spz.State(4)
+    with pytest.raises(spz.SpinozaError):
+        spz.apply(spz.Gate.H, s, 4)
+    with pytest.raises(spz.SpinozaError):
+        spz.c_apply(spz.Gate.X, s, 2, 2)
+    with pytest.raises(spz.SpinozaError):
+        spz.apply(spz.Gate.SWAP(0, 9), s, 0)
+    with pytest.raises(ValueError):
+        spz.State(0)
+    with pytest.raises(spz.SpinozaError) as e:
+        spz.measure_qubit(s, 0, True, 1)  # |0000>: outcome 1 has probability 0
+    assert e.value.code == 6
+    with pytest.raises(spz.SpinozaError) as e:
+        spz.State(40)  # 16 TiB: allocation must fail cleanly
+    assert e.value.code == 3
+    s2 = spz.State(4)
+    assert s2.norm() == 1.0
+
+
+def test_fp32_storage_fidelity(spz, orc):
+    print("case fp32", flush=True)
+    from spinoza_amd import _native as N
+    from spinoza_amd.executor import run_ops
+    from spinoza_amd.workloads import random_circuit
+    n = 16
+    ops = random_circuit(n, 8, 5)
+    ref = orc.OracleState(n)
+    orc.execute(ref, ops)
+    for mode in (N.EXEC_PLAIN, N.EXEC_AUTO):
+        s = spz.State(n, "f32")
+        run_ops(s, ops, mode=mode)
+        psi = s.to_numpy()
+        fid = abs(np.vdot(ref.re + 1j * ref.im, psi)) ** 2
+        assert 1 - fid < 1e-9, fid
+        assert abs(s.norm() - 1.0) < 1e-5
+        assert abs(spz.expect_zz(s, 0, n - 1) - ref.expect_zz(0, n - 1)) < 1e-5
+
+
+def test_full_size_properties(spz):
+    """At the 30-qubit workload size: H on every qubit gives the uniform
+    state exactly, X on every qubit maps |0> to |2^n - 1>, norm stays 1."""
+    print("case 30q properties", flush=True)
+    n = 30
+    s = spz.State(n)
+    for q in range(n):
+        spz.apply(spz.Gate.H, s, q)
+    re, im = s.download(0, 4096)
+    assert np.all(re == re[0]) and np.all(im == 0.0)
+    assert abs(re[0] - 2.0 ** (-15)) < 1e-18
+    assert abs(s.norm() - 1.0) < 1e-13
+    s.set_basis(0)
+    for q in range(n):
+        spz.apply(spz.Gate.X, s, q)
+    assert s[(1 << n) - 1] == (1.0, 0.0) and s[0] == (0.0, 0.0)
+    del s
