@@ -66,6 +66,7 @@ def per_stride(spz, N, n):
         "CX(c=t+1 mod n)": gate_bytes({"kind": N.X, "ones": 1}, n),
     }
     out = {}
+    N.check(N.lib().spz_profile_enable(1))
     for name, fn in classes.items():
         row = []
         for t in range(n):
@@ -81,8 +82,9 @@ def per_stride(spz, N, n):
                 ms = ctypes.c_double()
                 N.check(N.lib().spz_profile_get(cls, None, ctypes.byref(ms), None))
                 tot += ms.value
-            row.append(round(byts[name] * reps / (tot * 1e-3) / 1e9, 1))
+            row.append(round(byts[name] * reps / (tot * 1e-3) / 1e9, 1) if tot > 0 else None)
         out[name] = row
+    N.check(N.lib().spz_profile_enable(0))
     del s
     return out
 

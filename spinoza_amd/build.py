@@ -31,6 +31,16 @@ COMMON = [
 ]
 
 
+# Per-file flags.  The tile kernel selects the target's register slot with a
+# runtime switch over template instances; LLVM's SimplifyCFG would sink the
+# near-identical case bodies into one block with a computed register index,
+# which forces the amplitude arrays into scratch memory (272 B/lane, ~25x
+# slower).  Keeping the cases apart keeps them in VGPRs.
+EXTRA = {
+    "spz_tile.hip": ["-mllvm", "-simplifycfg-sink-common=false"],
+}
+
+
 def hipcc():
     for cand in (os.environ.get("HIPCC"), "/opt/rocm/bin/hipcc", shutil.which("hipcc")):
         if cand and os.path.exists(cand):
@@ -44,7 +54,7 @@ def sources():
 
 def _compile(src):
     obj = os.path.join(BUILD, os.path.basename(src) + ".o")
-    cmd = [hipcc(), "-c", src, "-o", obj] + COMMON
+    cmd = [hipcc(), "-c", src, "-o", obj] + COMMON + EXTRA.get(os.path.basename(src), [])
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"hipcc failed on {src}:\n{r.stdout}\n{r.stderr}")

@@ -1,7 +1,23 @@
 // spz_exec.hip -- QuantumCircuit::execute over an op list (circuit.rs:552-600).
+//
+// PLAIN  : one launch per gate, in program order (the reference's loop).
+// STRICT : runs of consecutive gates are packed into fused tile sweeps
+//          without reordering anything, so results are bit-identical to PLAIN.
+// AUTO   : like STRICT, but a gate may be pulled ahead of earlier gates it
+//          commutes with (disjoint qubits, or qubits on which both act
+//          diagonally: Z-type targets and controls), which packs more gates
+//          into each sweep; floating-point results then differ from PLAIN only
+//          by rounding (the same products in a different order).
+//
+// Measurements, SWAPs and dense blocks end a run and execute on their own.
+#include <stdlib.h>
 #include <string.h>
 
+#include <algorithm>
+#include <vector>
+
 #include "spz_internal.h"
+#include "spz_tile.h"
 
 namespace spz {
 
@@ -56,6 +72,274 @@ int exec_one_plain(spz_state_s *s, const spz_op &o, spz_tracker *trk, uint8_t *o
     return apply_gate(s, o.kind, o.theta, o.phi, o.lambda, o.target, o.t1, o.ones, o.zeros);
 }
 
+// A single-target gate of a fusable run.
+struct SG {
+    int op_index;
+    int kind;
+    int target;
+    uint64_t ones, zeros;
+    Coef k;
+    bool diag;
+    uint64_t nd;  // qubits acted on non-diagonally (the target of a 2x2 gate)
+    uint64_t dm;  // qubits acted on diagonally (controls, diagonal targets)
+};
+
+bool is_barrier(const spz_op &o)
+{
+    return o.kind == SPZ_M || o.kind == SPZ_SWAP || o.kind == SPZ_UNITARY || o.kind < SPZ_H ||
+           o.kind > SPZ_BITFLIP;
+}
+
+// Validate and lower one op of a run to an SG; returns 1 if it is a no-op.
+int lower(const spz_state_s *s, const spz_op &o, const spz_tracker *trk, int idx, SG *g)
+{
+    int kind = o.kind;
+    uint64_t ones = o.ones, zeros = o.zeros;
+    if (kind == SPZ_BITFLIP) {
+        if (ones || zeros) return fail(SPZ_ERR_GATE, "controlled bit-flip noise");
+        if (!(o.u <= o.theta)) return -1;  // no flip: nothing to do
+        kind = SPZ_X;
+    }
+    if ((o.flags & SPZ_OPF_CLASSICAL) && zeros == 0 && __builtin_popcountll(ones) == 1 && trk) {
+        int c = __builtin_ctzll(ones);
+        if ((trk->measured >> c) & 1) {
+            if (!((trk->values >> c) & 1)) return -1;
+            ones = 0;
+        }
+    }
+    const int t = o.target;
+    if (t < 0 || t >= s->n) return fail(SPZ_ERR_QUBIT, "target qubit out of range");
+    const uint64_t tb = 1ull << t;
+    if ((ones | zeros) & tb) return fail(SPZ_ERR_QUBIT, "target is also a control");
+    if (ones & zeros) return fail(SPZ_ERR_QUBIT, "qubit both a 1-control and a 0-control");
+    if (s->n < 64 && ((ones | zeros) >> s->n)) return fail(SPZ_ERR_QUBIT, "control out of range");
+    g->op_index = idx;
+    g->kind = kind;
+    g->target = t;
+    g->ones = ones;
+    g->zeros = zeros;
+    SPZ_TRY(make_coef(kind, o.theta, o.phi, o.lambda, &g->k));
+    g->diag = is_diagonal(g->k.ak);
+    g->nd = g->diag ? 0 : tb;
+    g->dm = (ones | zeros) | (g->diag ? tb : 0);
+    return SPZ_OK;
+}
+
+int env_int(const char *name, int dflt)
+{
+    const char *e = getenv(name);
+    return (e && *e) ? atoi(e) : dflt;
+}
+
+struct Sweep {
+    bool tile;                 // false: a single gate on the plain path
+    int gate;                  // plain: index into the run
+    TileDesc desc;
+    size_t op_begin, op_count; // range in the program buffer
+};
+
+struct Planner {
+    int n, K, LOGW, RB, klo;
+    bool strict;
+    std::vector<TileOp> prog;
+    std::vector<Sweep> sweeps;
+
+    // Choose the tile qubits and the gates of one sweep from `rem` (indices
+    // into `run`, in program order).  Returns the chosen gate indices.
+    std::vector<int> collect(const std::vector<SG> &run, const std::vector<int> &rem,
+                             uint64_t &smask)
+    {
+        std::vector<int> order;
+        smask = (klo >= 64) ? ~0ull : ((1ull << klo) - 1);
+        int free_slots = K - klo;
+        uint64_t bnd = 0, bd = 0;
+        const uint64_t all = n >= 64 ? ~0ull : ((1ull << n) - 1);
+        for (int idx : rem) {
+            const SG &g = run[idx];
+            bool ok = !(g.nd & (bnd | bd)) && !(g.dm & bnd);
+            if (ok && g.nd && !(g.nd & smask)) {
+                if (free_slots > 0) {
+                    smask |= g.nd;
+                    free_slots--;
+                } else {
+                    ok = false;
+                }
+            }
+            if (ok) {
+                order.push_back(idx);
+            } else {
+                if (strict) break;
+                bnd |= g.nd;
+                bd |= g.dm;
+                if ((bnd & all) == all) break;
+            }
+        }
+        for (int q = 0; q < n && __builtin_popcountll(smask) < K; q++) smask |= 1ull << q;
+        return order;
+    }
+
+    void emit_layout(const std::vector<int> &RSv)
+    {
+        TileOp op;
+        memset(&op, 0, sizeof op);
+        op.type = TOP_LAYOUT;
+        uint32_t used = 0;
+        for (int j = 0; j < RB; j++) {
+            op.regbits[j] = (int8_t)RSv[j];
+            used |= 1u << RSv[j];
+        }
+        int i = 0;
+        for (int b = 0; b < K; b++)
+            if (!((used >> b) & 1)) op.thrbits[i++] = (int8_t)b;
+        prog.push_back(op);
+    }
+
+    void build(const std::vector<SG> &run, const std::vector<int> &order, uint64_t smask)
+    {
+        Sweep sw;
+        memset(&sw.desc, 0, sizeof sw.desc);
+        sw.tile = true;
+        sw.gate = -1;
+        int lbit[64];
+        for (int q = 0; q < 64; q++) lbit[q] = -1;
+        int i = 0;
+        for (int q = 0; q < n; q++)
+            if ((smask >> q) & 1) {
+                sw.desc.spos[i] = (int8_t)q;
+                lbit[q] = i++;
+            }
+        for (; i < 16; i++) sw.desc.spos[i] = 64;
+        for (int p = 0; p < (1 << RB); p++) {
+            uint64_t g = 0;
+            for (int j = 0; j < RB; j++)
+                if ((p >> j) & 1) g |= 1ull << sw.desc.spos[LOGW + j];
+            sw.desc.gr[p] = g;
+        }
+        sw.op_begin = prog.size();
+        std::vector<int> L0(RB), RS(RB);
+        for (int j = 0; j < RB; j++) L0[j] = RS[j] = LOGW + j;
+        for (size_t pos = 0; pos < order.size(); pos++) {
+            const SG &g = run[order[pos]];
+            TileOp op;
+            memset(&op, 0, sizeof op);
+            op.ak = g.k.ak;
+            for (int j = 0; j < 8; j++) op.c[j] = g.k.c[j];
+            for (int q = 0; q < n; q++) {
+                if ((g.ones >> q) & 1) {
+                    if (lbit[q] >= 0) op.lc |= 1u << lbit[q]; else op.eo |= 1ull << q;
+                }
+                if ((g.zeros >> q) & 1) {
+                    if (lbit[q] >= 0) op.lz |= 1u << lbit[q]; else op.ez |= 1ull << q;
+                }
+            }
+            if (g.diag) {
+                op.type = TOP_DIAG;
+                op.dbit = lbit[g.target];
+                op.ebit = op.dbit < 0 ? (1ull << g.target) : 0;
+            } else {
+                const int b = lbit[g.target];
+                int slot = -1;
+                for (int j = 0; j < RB; j++)
+                    if (RS[j] == b) slot = j;
+                if (slot < 0) {
+                    // next register set: upcoming non-diagonal targets in first-use order
+                    std::vector<int> nx;
+                    for (size_t p2 = pos; p2 < order.size() && (int)nx.size() < RB; p2++) {
+                        const SG &h = run[order[p2]];
+                        if (h.diag) continue;
+                        int hb = lbit[h.target];
+                        if (std::find(nx.begin(), nx.end(), hb) == nx.end()) nx.push_back(hb);
+                    }
+                    for (int j = 0; j < RB && (int)nx.size() < RB; j++)
+                        if (std::find(nx.begin(), nx.end(), RS[j]) == nx.end()) nx.push_back(RS[j]);
+                    for (int bb = 0; bb < K && (int)nx.size() < RB; bb++)
+                        if (std::find(nx.begin(), nx.end(), bb) == nx.end()) nx.push_back(bb);
+                    RS = nx;
+                    emit_layout(RS);
+                    for (int j = 0; j < RB; j++)
+                        if (RS[j] == b) slot = j;
+                }
+                op.type = TOP_PAIR;
+                op.slot = slot;
+            }
+            prog.push_back(op);
+        }
+        if (RS != L0) emit_layout(L0);
+        sw.op_count = prog.size() - sw.op_begin;
+        sw.desc.nops = (int32_t)sw.op_count;
+        sweeps.push_back(sw);
+    }
+
+    void plan(const std::vector<SG> &run)
+    {
+        std::vector<int> rem(run.size());
+        for (size_t i = 0; i < run.size(); i++) rem[i] = (int)i;
+        while (!rem.empty()) {
+            uint64_t smask;
+            std::vector<int> order = collect(run, rem, smask);
+            if (order.size() < 2) {
+                // nothing to fuse: the first remaining gate runs on the plain path
+                Sweep sw;
+                memset(&sw, 0, sizeof sw);
+                sw.tile = false;
+                sw.gate = rem[0];
+                sweeps.push_back(sw);
+                rem.erase(rem.begin());
+                continue;
+            }
+            build(run, order, smask);
+            std::vector<int> nrem;
+            nrem.reserve(rem.size());
+            size_t k = 0;
+            for (int idx : rem) {
+                if (k < order.size() && order[k] == idx) { k++; continue; }
+                nrem.push_back(idx);
+            }
+            rem.swap(nrem);
+        }
+    }
+};
+
+// Device program buffer (grown as needed, reused across calls).
+TileOp *g_dprog = nullptr;
+size_t g_dprog_cap = 0;
+
+int run_fused(spz_state_s *s, const std::vector<SG> &run, int K, bool strict)
+{
+    Planner pl;
+    pl.n = s->n;
+    pl.K = K;
+    pl.LOGW = K - 4;
+    pl.RB = 4;
+    pl.klo = std::min(env_int("SPZ_TILE_KLO", 4), K);
+    pl.strict = strict;
+    pl.plan(run);
+    if (!pl.prog.empty()) {
+        SPZ_HIP(hipStreamSynchronize(rt().stream));  // the previous program may still be in use
+        if (pl.prog.size() > g_dprog_cap) {
+            if (g_dprog) SPZ_HIP(hipFree(g_dprog));
+            g_dprog = nullptr;
+            size_t cap = std::max(pl.prog.size(), (size_t)4096);
+            if (hipMalloc(&g_dprog, cap * sizeof(TileOp)) != hipSuccess)
+                return fail(SPZ_ERR_OOM, "tile program allocation failed");
+            g_dprog_cap = cap;
+        }
+        SPZ_HIP(hipMemcpyAsync(g_dprog, pl.prog.data(), pl.prog.size() * sizeof(TileOp),
+                               hipMemcpyHostToDevice, rt().stream));
+    }
+    for (const Sweep &sw : pl.sweeps) {
+        if (sw.tile) {
+            SPZ_TRY(launch_tile(s, K, sw.desc, g_dprog + sw.op_begin));
+        } else {
+            const SG &g = run[sw.gate];
+            if (g.diag) SPZ_TRY(launch_diag(s, g.k, g.target, g.ones, g.zeros));
+            else SPZ_TRY(launch_pair(s, g.k, g.target, g.ones, g.zeros));
+        }
+    }
+    if (!pl.prog.empty()) SPZ_HIP(hipStreamSynchronize(rt().stream));  // program buffer reuse
+    return SPZ_OK;
+}
+
 }  // namespace
 
 int execute_plain(spz_state_s *s, const spz_op *ops, uint64_t n_ops, spz_tracker *trk,
@@ -74,8 +358,44 @@ int execute_plain(spz_state_s *s, const spz_op *ops, uint64_t n_ops, spz_tracker
 int execute_ops(spz_state_s *s, const spz_op *ops, uint64_t n_ops, spz_tracker *trk,
                 const spz_exec_opts *opts, uint8_t *outcomes)
 {
-    (void)opts;
-    return execute_plain(s, ops, n_ops, trk, outcomes);
+    int mode = opts ? opts->mode : SPZ_EXEC_AUTO;
+    if (mode == SPZ_EXEC_PLAIN) return execute_plain(s, ops, n_ops, trk, outcomes);
+    if (mode != SPZ_EXEC_AUTO && mode != SPZ_EXEC_STRICT)
+        return fail(SPZ_ERR_ARG, "unknown execution mode");
+    int K = (opts && opts->tile_qubits) ? opts->tile_qubits : env_int("SPZ_TILE_K", 12);
+    if (K < 10 || K > 13) return fail(SPZ_ERR_ARG, "tile_qubits must be in [10, 13]");
+    if (s->n < K + 1) K = s->n - 1;
+    if (K < 10) return execute_plain(s, ops, n_ops, trk, outcomes);
+    const bool strict = mode == SPZ_EXEC_STRICT;
+    if (outcomes)
+        for (uint64_t i = 0; i < n_ops; i++) outcomes[i] = 255;
+    uint64_t i = 0;
+    while (i < n_ops) {
+        if (is_barrier(ops[i])) {
+            int rc = exec_one_plain(s, ops[i], trk, outcomes ? &outcomes[i] : nullptr);
+            if (rc != SPZ_OK) {
+                set_error("op " + std::to_string(i) + ": " + spz_last_error());
+                return rc;
+            }
+            i++;
+            continue;
+        }
+        std::vector<SG> run;
+        uint64_t j = i;
+        for (; j < n_ops && !is_barrier(ops[j]); j++) {
+            SG g;
+            int rc = lower(s, ops[j], trk, (int)j, &g);
+            if (rc == -1) continue;  // no-op (unflipped noise, classical control read 0)
+            if (rc != SPZ_OK) {
+                set_error("op " + std::to_string(j) + ": " + spz_last_error());
+                return rc;
+            }
+            run.push_back(g);
+        }
+        if (!run.empty()) SPZ_TRY(run_fused(s, run, K, strict));
+        i = j;
+    }
+    return SPZ_OK;
 }
 
 }  // namespace spz

@@ -1,0 +1,51 @@
+// spz_tile.h -- fused tile sweeps: data shared by the scheduler and the kernel.
+//
+// A sweep reads every amplitude once, applies a run of gates, and writes it
+// back.  The state is cut into 2^(n-K) tiles of 2^K amplitudes; a tile is
+// the set of indices that agree on every qubit outside the tile qubit set S
+// (|S| = K, always containing the lowest qubits so that each tile is a union
+// of contiguous runs).  Local index bit i of a tile is qubit S[i].
+//
+// A workgroup of W = 2^LOGW threads owns one tile; each thread keeps
+// R = 2^(K-LOGW) amplitudes in registers.  Which local bits are "register
+// bits" (the RB = K-LOGW bits of the register index) is the current layout;
+// a gate whose target is a register bit is a pure in-register pair update,
+// with controls and diagonal phases evaluated from each amplitude's local
+// index (or, for qubits outside S, once per tile).  Layout changes go through
+// LDS.  Loads and stores use the load layout (register bits = top RB local
+// bits), so lanes of a wavefront read contiguous runs.
+#pragma once
+
+#include <stdint.h>
+
+namespace spz {
+
+enum TileOpType : int32_t { TOP_PAIR = 0, TOP_DIAG = 1, TOP_LAYOUT = 2 };
+
+struct TileOp {
+    int32_t type;     // TileOpType
+    int32_t ak;       // Arith kind
+    int32_t slot;     // PAIR: register-index bit of the target
+    int32_t dbit;     // DIAG: local bit of the target, -1 if the target is outside S
+    uint32_t lc, lz;  // local control masks (must read 1 / 0)
+    uint32_t pad0, pad1;
+    uint64_t eo, ez;  // controls outside S (global index bits), tile-uniform
+    uint64_t ebit;    // DIAG with the target outside S: its global bit
+    double c[8];      // coefficients
+    int8_t regbits[8];   // LAYOUT: local bit of register-index bit j
+    int8_t thrbits[16];  // LAYOUT: local bit of thread-index bit i
+};
+
+constexpr int kTileMaxK = 14;
+constexpr int kTileMaxR = 32;
+
+struct TileDesc {
+    int8_t spos[16];          // qubit of local bit i (ascending), K entries
+    uint64_t gr[kTileMaxR];   // global offset of register index p in the load layout
+    int32_t nops;
+    int32_t pad;
+};
+
+int launch_tile(spz_state_s *s, int K, const TileDesc &d, const TileOp *d_ops);
+
+}  // namespace spz
