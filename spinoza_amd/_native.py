@@ -8,7 +8,7 @@ import ctypes
 import os
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "libspinoza_amd.so")
+LIB_PATH = os.environ.get("SPINOZA_AMD_LIB") or os.path.join(HERE, "libspinoza_amd.so")
 
 # status codes (enum spz_status)
 OK, ERR_QUBIT, ERR_GATE, ERR_OOM, ERR_HIP, ERR_ARG, ERR_PROB, ERR_NODEVICE = range(8)

@@ -141,6 +141,7 @@ struct Sweep {
 struct Planner {
     int n, K, LOGW, RB, klo;
     bool strict;
+    bool bad = false;
     std::vector<TileOp> prog;
     std::vector<Sweep> sweeps;
 
@@ -178,20 +179,121 @@ struct Planner {
         return order;
     }
 
-    void emit_layout(const std::vector<int> &RSv)
+    void emit_layout(const std::vector<int> &oldRS, const std::vector<int> &RSv)
     {
         TileOp op;
         memset(&op, 0, sizeof op);
         op.type = TOP_LAYOUT;
         uint32_t used = 0;
         for (int j = 0; j < RB; j++) {
-            op.regbits[j] = (int8_t)RSv[j];
+            op.oldreg[j] = (int8_t)oldRS[j];
+            op.newreg[j] = (int8_t)RSv[j];
             used |= 1u << RSv[j];
         }
         int i = 0;
         for (int b = 0; b < K; b++)
-            if (!((used >> b) & 1)) op.thrbits[i++] = (int8_t)b;
+            if (!((used >> b) & 1)) op.newthr[i++] = (int8_t)b;
         prog.push_back(op);
+    }
+
+    // Sequence the sweep's gates: the scheduled order itself (strict), or,
+    // with reordering allowed, a topological order of the dependency graph
+    // that applies every gate the current register layout allows before
+    // switching layouts.  Entries >= 0 are gates; -1 marks "switch layout to
+    // the next entry of `layouts`".
+    void sequence(const std::vector<SG> &run, const std::vector<int> &order, const int *lbit,
+                  std::vector<int> &seq, std::vector<std::vector<int>> &layouts)
+    {
+        const int m = (int)order.size();
+        std::vector<int> RS(RB);
+        for (int j = 0; j < RB; j++) RS[j] = LOGW + j;
+        auto in_rs = [&](int b) {
+            for (int j = 0; j < RB; j++)
+                if (RS[j] == b) return true;
+            return false;
+        };
+        if (strict) {
+            for (int pos = 0; pos < m; pos++) {
+                const SG &g = run[order[pos]];
+                if (!g.diag && !in_rs(lbit[g.target])) {
+                    std::vector<int> nx;
+                    for (int p2 = pos; p2 < m && (int)nx.size() < RB; p2++) {
+                        const SG &h = run[order[p2]];
+                        if (h.diag) continue;
+                        int hb = lbit[h.target];
+                        if (std::find(nx.begin(), nx.end(), hb) == nx.end()) nx.push_back(hb);
+                    }
+                    for (int j = 0; j < RB && (int)nx.size() < RB; j++)
+                        if (std::find(nx.begin(), nx.end(), RS[j]) == nx.end()) nx.push_back(RS[j]);
+                    for (int bb = 0; bb < K && (int)nx.size() < RB; bb++)
+                        if (std::find(nx.begin(), nx.end(), bb) == nx.end()) nx.push_back(bb);
+                    RS = nx;
+                    seq.push_back(-1);
+                    layouts.push_back(RS);
+                }
+                seq.push_back(order[pos]);
+            }
+            return;
+        }
+        // dependency graph: i -> j (i before j) when they share a qubit on
+        // which at least one acts non-diagonally
+        std::vector<std::vector<int>> succ(m);
+        std::vector<int> indeg(m, 0);
+        for (int a = 0; a < m; a++) {
+            const SG &x = run[order[a]];
+            for (int b = a + 1; b < m; b++) {
+                const SG &y = run[order[b]];
+                if ((x.nd & (y.nd | y.dm)) || (x.dm & y.nd)) {
+                    succ[a].push_back(b);
+                    indeg[b]++;
+                }
+            }
+        }
+        std::vector<char> done(m, 0);
+        int left = m;
+        while (left > 0) {
+            bool progress = true;
+            while (progress) {
+                progress = false;
+                for (int a = 0; a < m; a++) {
+                    if (done[a] || indeg[a]) continue;
+                    const SG &g = run[order[a]];
+                    if (!g.diag && !in_rs(lbit[g.target])) continue;
+                    seq.push_back(order[a]);
+                    done[a] = 1;
+                    left--;
+                    for (int b : succ[a]) indeg[b]--;
+                    progress = true;
+                }
+            }
+            if (left == 0) break;
+            // next layout: score target bits of the remaining non-diagonal
+            // gates, ready ones first
+            std::vector<long> score(K, 0);
+            for (int a = 0; a < m; a++) {
+                if (done[a]) continue;
+                const SG &g = run[order[a]];
+                if (g.diag) continue;
+                score[lbit[g.target]] += indeg[a] == 0 ? 1000 : 1;
+            }
+            std::vector<int> bits(K);
+            for (int b = 0; b < K; b++) bits[b] = b;
+            std::stable_sort(bits.begin(), bits.end(),
+                             [&](int x, int y) { return score[x] > score[y]; });
+            std::vector<int> nx(bits.begin(), bits.begin() + RB);
+            // keep bits that stay in place in their old slots
+            std::vector<int> slotted(RB, -1);
+            for (int j = 0; j < RB; j++)
+                if (std::find(nx.begin(), nx.end(), RS[j]) != nx.end()) slotted[j] = RS[j];
+            for (int b : nx) {
+                if (std::find(slotted.begin(), slotted.end(), b) != slotted.end()) continue;
+                for (int j = 0; j < RB; j++)
+                    if (slotted[j] < 0) { slotted[j] = b; break; }
+            }
+            RS = slotted;
+            seq.push_back(-1);
+            layouts.push_back(RS);
+        }
     }
 
     void build(const std::vector<SG> &run, const std::vector<int> &order, uint64_t smask)
@@ -216,55 +318,68 @@ struct Planner {
             sw.desc.gr[p] = g;
         }
         sw.op_begin = prog.size();
+        std::vector<int> seq;
+        std::vector<std::vector<int>> layouts;
+        sequence(run, order, lbit, seq, layouts);
         std::vector<int> L0(RB), RS(RB);
         for (int j = 0; j < RB; j++) L0[j] = RS[j] = LOGW + j;
-        for (size_t pos = 0; pos < order.size(); pos++) {
-            const SG &g = run[order[pos]];
+        size_t li_next = 0;
+        for (int gi : seq) {
+            if (gi < 0) {
+                const std::vector<int> &nx = layouts[li_next++];
+                if (nx != RS) emit_layout(RS, nx);
+                RS = nx;
+                continue;
+            }
+            const SG &g = run[gi];
             TileOp op;
             memset(&op, 0, sizeof op);
             op.ak = g.k.ak;
             for (int j = 0; j < 8; j++) op.c[j] = g.k.c[j];
+            auto slot_of = [&](int b) {
+                for (int j = 0; j < RB; j++)
+                    if (RS[j] == b) return j;
+                return -1;
+            };
             for (int q = 0; q < n; q++) {
-                if ((g.ones >> q) & 1) {
-                    if (lbit[q] >= 0) op.lc |= 1u << lbit[q]; else op.eo |= 1ull << q;
+                const bool one = (g.ones >> q) & 1, zero = (g.zeros >> q) & 1;
+                if (!one && !zero) continue;
+                const int b = lbit[q];
+                if (b < 0) {
+                    if (one) op.eo |= 1ull << q; else op.ez |= 1ull << q;
+                    continue;
                 }
-                if ((g.zeros >> q) & 1) {
-                    if (lbit[q] >= 0) op.lz |= 1u << lbit[q]; else op.ez |= 1ull << q;
+                const int j = slot_of(b);
+                if (j >= 0) {
+                    if (one) op.mro |= 1u << j; else op.mrz |= 1u << j;
+                } else {
+                    if (one) op.mlo |= 1u << b; else op.mlz |= 1u << b;
                 }
             }
             if (g.diag) {
                 op.type = TOP_DIAG;
-                op.dbit = lbit[g.target];
-                op.ebit = op.dbit < 0 ? (1ull << g.target) : 0;
-            } else {
                 const int b = lbit[g.target];
-                int slot = -1;
-                for (int j = 0; j < RB; j++)
-                    if (RS[j] == b) slot = j;
-                if (slot < 0) {
-                    // next register set: upcoming non-diagonal targets in first-use order
-                    std::vector<int> nx;
-                    for (size_t p2 = pos; p2 < order.size() && (int)nx.size() < RB; p2++) {
-                        const SG &h = run[order[p2]];
-                        if (h.diag) continue;
-                        int hb = lbit[h.target];
-                        if (std::find(nx.begin(), nx.end(), hb) == nx.end()) nx.push_back(hb);
-                    }
-                    for (int j = 0; j < RB && (int)nx.size() < RB; j++)
-                        if (std::find(nx.begin(), nx.end(), RS[j]) == nx.end()) nx.push_back(RS[j]);
-                    for (int bb = 0; bb < K && (int)nx.size() < RB; bb++)
-                        if (std::find(nx.begin(), nx.end(), bb) == nx.end()) nx.push_back(bb);
-                    RS = nx;
-                    emit_layout(RS);
-                    for (int j = 0; j < RB; j++)
-                        if (RS[j] == b) slot = j;
+                if (b < 0) {
+                    op.dsel = 2;
+                    op.ebit = 1ull << g.target;
+                } else if (slot_of(b) >= 0) {
+                    op.dsel = 0;
+                    op.dpos = slot_of(b);
+                } else {
+                    op.dsel = 1;
+                    op.dpos = b;
                 }
+            } else {
                 op.type = TOP_PAIR;
-                op.slot = slot;
+                op.slot = slot_of(lbit[g.target]);
+                if (op.slot < 0) {  // sequence() switches layouts before such a gate
+                    bad = true;
+                    return;
+                }
             }
             prog.push_back(op);
         }
-        if (RS != L0) emit_layout(L0);
+        if (RS != L0) emit_layout(RS, L0);
         sw.op_count = prog.size() - sw.op_begin;
         sw.desc.nops = (int32_t)sw.op_count;
         sweeps.push_back(sw);
@@ -314,6 +429,7 @@ int run_fused(spz_state_s *s, const std::vector<SG> &run, int K, bool strict)
     pl.klo = std::min(env_int("SPZ_TILE_KLO", 4), K);
     pl.strict = strict;
     pl.plan(run);
+    if (pl.bad) return fail(SPZ_ERR_ARG, "tile planner produced a gate outside the register layout");
     if (!pl.prog.empty()) {
         SPZ_HIP(hipStreamSynchronize(rt().stream));  // the previous program may still be in use
         if (pl.prog.size() > g_dprog_cap) {
@@ -399,11 +515,3 @@ int execute_ops(spz_state_s *s, const spz_op *ops, uint64_t n_ops, spz_tracker *
 }
 
 }  // namespace spz
-
-extern "C" int spz_adjoint_grad(spz_state s, const spz_op *ops, uint64_t n_ops, double hx,
-                                int ckpt_every, double *grad, double *energy, double *drift)
-{
-    (void)s; (void)ops; (void)n_ops; (void)hx; (void)ckpt_every; (void)grad; (void)energy;
-    (void)drift;
-    return spz::fail(SPZ_ERR_GATE, "adjoint gradient not built yet");
-}

@@ -26,14 +26,18 @@ struct TileOp {
     int32_t type;     // TileOpType
     int32_t ak;       // Arith kind
     int32_t slot;     // PAIR: register-index bit of the target
-    int32_t dbit;     // DIAG: local bit of the target, -1 if the target is outside S
-    uint32_t lc, lz;  // local control masks (must read 1 / 0)
-    uint32_t pad0, pad1;
-    uint64_t eo, ez;  // controls outside S (global index bits), tile-uniform
-    uint64_t ebit;    // DIAG with the target outside S: its global bit
-    double c[8];      // coefficients
-    int8_t regbits[8];   // LAYOUT: local bit of register-index bit j
-    int8_t thrbits[16];  // LAYOUT: local bit of thread-index bit i
+    int32_t dsel;     // DIAG target: 0 register-index bit dpos, 1 local bit dpos of the
+                      // lane part, 2 outside S (global bit ebit)
+    int32_t dpos;
+    uint32_t mro, mrz;  // controls on register-index bits (must read 1 / 0): uniform
+    uint32_t mlo, mlz;  // controls on local bits held by the lane part (per lane)
+    uint32_t pad;
+    uint64_t eo, ez;    // controls outside S (global index bits), uniform per tile
+    uint64_t ebit;      // DIAG with the target outside S: its global bit
+    double c[8];        // coefficients
+    int8_t oldreg[8];   // LAYOUT: local bit of register-index bit j before ...
+    int8_t newreg[8];   // ... and after the change
+    int8_t newthr[16];  // LAYOUT: local bit of thread-index bit i after the change
 };
 
 constexpr int kTileMaxK = 14;

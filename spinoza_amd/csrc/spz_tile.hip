@@ -20,54 +20,137 @@ namespace {
 // spread over the banks (a bijection inside every aligned block of 16).
 __device__ __forceinline__ uint32_t swz(uint32_t l) { return l ^ ((l >> 4) & 15u) ^ ((l >> 8) & 15u); }
 
-template <int R, int AK, int J>
-__device__ __forceinline__ void pair_j(double *vr, double *vi, const uint32_t *li,
-                                       const double *c, uint32_t lc, uint32_t lz)
+// In-register pair update on register bit J.  PRED: some controls sit on
+// register bits; the test on the constant p is wavefront-uniform, and the
+// update is committed with a select so that no control flow splits the
+// register arrays.
+template <int R, int AK, int J, bool PRED>
+__device__ __forceinline__ void pair_j(double *vr, double *vi, const double *k, uint32_t mro,
+                                       uint32_t mrz)
 {
-    if (lc | lz) {
 #pragma unroll
-        for (int p = 0; p < R; p++) {
-            if (p & (1 << J)) continue;
-            const int q = p | (1 << J);
-            const uint32_t l = li[p];
-            if ((l & lc) == lc && !(l & lz)) pair_update(AK, c, vr[p], vi[p], vr[q], vi[q]);
-        }
-    } else {
-#pragma unroll
-        for (int p = 0; p < R; p++) {
-            if (p & (1 << J)) continue;
-            const int q = p | (1 << J);
-            pair_update(AK, c, vr[p], vi[p], vr[q], vi[q]);
+    for (int p = 0; p < R; p++) {
+        if (p & (1 << J)) continue;
+        const int q = p | (1 << J);
+        if constexpr (!PRED) {
+            pair_update(AK, k, vr[p], vi[p], vr[q], vi[q]);
+        } else {
+            const bool ok = ((uint32_t)p & mro) == mro && !((uint32_t)p & mrz);
+            double a = vr[p], b = vi[p], c = vr[q], d = vi[q];
+            pair_update(AK, k, a, b, c, d);
+            vr[p] = ok ? a : vr[p];
+            vi[p] = ok ? b : vi[p];
+            vr[q] = ok ? c : vr[q];
+            vi[q] = ok ? d : vi[q];
         }
     }
 }
 
 template <int R, int RB, int AK>
-__device__ __forceinline__ void pair_slot(int slot, double *vr, double *vi, const uint32_t *li,
-                                          const double *c, uint32_t lc, uint32_t lz)
+__device__ __forceinline__ void pair_slot(int slot, double *vr, double *vi, const double *k,
+                                          uint32_t mro, uint32_t mrz)
 {
-    switch (slot) {
-    case 0: pair_j<R, AK, 0>(vr, vi, li, c, lc, lz); break;
-    case 1: if constexpr (RB > 1) pair_j<R, AK, 1>(vr, vi, li, c, lc, lz); break;
-    case 2: if constexpr (RB > 2) pair_j<R, AK, 2>(vr, vi, li, c, lc, lz); break;
-    case 3: if constexpr (RB > 3) pair_j<R, AK, 3>(vr, vi, li, c, lc, lz); break;
-    case 4: if constexpr (RB > 4) pair_j<R, AK, 4>(vr, vi, li, c, lc, lz); break;
-    default: break;
+    if ((mro | mrz) == 0) {
+        switch (slot) {
+        case 0: pair_j<R, AK, 0, false>(vr, vi, k, 0, 0); break;
+        case 1: if constexpr (RB > 1) pair_j<R, AK, 1, false>(vr, vi, k, 0, 0); break;
+        case 2: if constexpr (RB > 2) pair_j<R, AK, 2, false>(vr, vi, k, 0, 0); break;
+        case 3: if constexpr (RB > 3) pair_j<R, AK, 3, false>(vr, vi, k, 0, 0); break;
+        default: break;
+        }
+    } else {
+        switch (slot) {
+        case 0: pair_j<R, AK, 0, true>(vr, vi, k, mro, mrz); break;
+        case 1: if constexpr (RB > 1) pair_j<R, AK, 1, true>(vr, vi, k, mro, mrz); break;
+        case 2: if constexpr (RB > 2) pair_j<R, AK, 2, true>(vr, vi, k, mro, mrz); break;
+        case 3: if constexpr (RB > 3) pair_j<R, AK, 3, true>(vr, vi, k, mro, mrz); break;
+        default: break;
+        }
     }
 }
 
-template <int R, int AK>
-__device__ __forceinline__ void diag_all(double *vr, double *vi, const uint32_t *li,
-                                         const double *c, int dbit, int ebitset, uint32_t lc,
-                                         uint32_t lz)
+// Diagonal update of one amplitude; `bit` is the target bit, the two
+// multipliers are values (never selected by address).
+template <int AK>
+__device__ __forceinline__ void diag_one(const double *k, bool bit, double &x, double &y)
+{
+    if constexpr (AK == AK_NEG) {  // z_proc_chunk: negate the |1> amplitude
+        const double nx = -x, ny = -y;
+        x = bit ? nx : x;
+        y = bit ? ny : y;
+    } else if constexpr (AK == AK_PHASE) {  // p_proc_chunk on the |1> amplitude
+        const double zr = x, zi = y;
+        const double nx = __builtin_fma(zr, k[0], -zi * k[1]);
+        const double ny = __builtin_fma(zi, k[0], zr * k[1]);
+        x = bit ? nx : x;
+        y = bit ? ny : y;
+    } else {  // AK_RZ: multiply by diag[bit]
+        const double mr = bit ? k[2] : k[0];
+        const double mi = bit ? k[3] : k[1];
+        const double cr = x, ci = y;
+        x = __builtin_fma(cr, mr, -ci * mi);
+        y = __builtin_fma(cr, mi, ci * mr);
+    }
+}
+
+// DSEL: 0 = target on register bit J (bit known per register), 1 = target on
+// a lane bit (lanebit), 2 = target outside the tile (ebit, uniform).
+template <int R, int AK, int DSEL, int J, bool PRED>
+__device__ __forceinline__ void diag_j(double *vr, double *vi, const double *k, bool lanebit,
+                                       uint32_t mro, uint32_t mrz)
 {
 #pragma unroll
     for (int p = 0; p < R; p++) {
-        const uint32_t l = li[p];
-        if ((l & lc) != lc || (l & lz)) continue;
-        const int bit = dbit >= 0 ? (int)((l >> dbit) & 1u) : ebitset;
-        diag_update(AK, c, bit, vr[p], vi[p]);
+        const bool bit = DSEL == 0 ? (((p >> J) & 1) != 0) : lanebit;
+        if constexpr (DSEL == 0 && !PRED && (AK == AK_NEG || AK == AK_PHASE)) {
+            if (!bit) continue;  // |0> amplitudes are untouched: skipped at compile time
+        }
+        if constexpr (!PRED) {
+            diag_one<AK>(k, bit, vr[p], vi[p]);
+        } else {
+            const bool ok = ((uint32_t)p & mro) == mro && !((uint32_t)p & mrz);
+            double x = vr[p], y = vi[p];
+            diag_one<AK>(k, bit, x, y);
+            vr[p] = ok ? x : vr[p];
+            vi[p] = ok ? y : vi[p];
+        }
     }
+}
+
+template <int R, int RB, int AK, bool PRED>
+__device__ __forceinline__ void diag_sel(double *vr, double *vi, const double *k, int dsel,
+                                         int dpos, bool lanebit, uint32_t mro, uint32_t mrz)
+{
+    if (dsel == 0) {
+        switch (dpos) {
+        case 0: diag_j<R, AK, 0, 0, PRED>(vr, vi, k, false, mro, mrz); break;
+        case 1: if constexpr (RB > 1) diag_j<R, AK, 0, 1, PRED>(vr, vi, k, false, mro, mrz); break;
+        case 2: if constexpr (RB > 2) diag_j<R, AK, 0, 2, PRED>(vr, vi, k, false, mro, mrz); break;
+        case 3: if constexpr (RB > 3) diag_j<R, AK, 0, 3, PRED>(vr, vi, k, false, mro, mrz); break;
+        default: break;
+        }
+    } else {
+        diag_j<R, AK, 1, 0, PRED>(vr, vi, k, lanebit, mro, mrz);
+    }
+}
+
+template <int R, int RB, int AK>
+__device__ __forceinline__ void diag_all(double *vr, double *vi, const double *k, int dsel,
+                                         int dpos, bool lanebit, uint32_t mro, uint32_t mrz)
+{
+    if ((mro | mrz) == 0) diag_sel<R, RB, AK, false>(vr, vi, k, dsel, dpos, lanebit, 0, 0);
+    else diag_sel<R, RB, AK, true>(vr, vi, k, dsel, dpos, lanebit, mro, mrz);
+}
+
+// Register part of the local index of register p under a layout.
+template <int RB>
+__device__ __forceinline__ uint32_t reg_part(int p, const int8_t *regbits)
+{
+    uint32_t r = 0;
+#pragma unroll
+    for (int j = 0; j < RB; j++)
+        if ((p >> j) & 1) r |= 1u << regbits[j];
+    return r;
 }
 
 template <typename T, int K, int LOGW>
@@ -87,75 +170,77 @@ k_tile(T *__restrict__ re, T *__restrict__ im, const TileDesc d, const TileOp *_
         if ((tid >> i) & 1u) gt |= 1ull << d.spos[i];
 
     double vr[R], vi[R];
-    uint32_t li[R];
 #pragma unroll
     for (int p = 0; p < R; p++) {
         const uint64_t g = base | gt | d.gr[p];
         vr[p] = (double)re[g];
         vi[p] = (double)im[g];
-        li[p] = ((uint32_t)p << LOGW) | tid;
     }
+    // lane part of the local index: in the load layout the thread index holds
+    // local bits 0..LOGW-1 and register p holds bits LOGW..K-1
+    uint32_t lt = tid;
 
     const int nops = d.nops;
     for (int o = 0; o < nops; o++) {
         const TileOp *op = ops + o;
         const int type = op->type;
         if (type == TOP_LAYOUT) {
-            uint32_t lt = 0;
+            uint32_t nlt = 0;
 #pragma unroll
             for (int i = 0; i < LOGW; i++)
-                if ((tid >> i) & 1u) lt |= 1u << op->thrbits[i];
-            uint32_t nl[R];
-#pragma unroll
-            for (int p = 0; p < R; p++) {
-                uint32_t lr = 0;
-#pragma unroll
-                for (int j = 0; j < RB; j++)
-                    if ((p >> j) & 1) lr |= 1u << op->regbits[j];
-                nl[p] = lt | lr;
-            }
+                if ((tid >> i) & 1u) nlt |= 1u << op->newthr[i];
             __syncthreads();
 #pragma unroll
-            for (int p = 0; p < R; p++) lds[swz(li[p])] = vr[p];
+            for (int p = 0; p < R; p++) lds[swz(lt | reg_part<RB>(p, op->oldreg))] = vr[p];
             __syncthreads();
 #pragma unroll
-            for (int p = 0; p < R; p++) vr[p] = lds[swz(nl[p])];
+            for (int p = 0; p < R; p++) vr[p] = lds[swz(nlt | reg_part<RB>(p, op->newreg))];
             __syncthreads();
 #pragma unroll
-            for (int p = 0; p < R; p++) lds[swz(li[p])] = vi[p];
+            for (int p = 0; p < R; p++) lds[swz(lt | reg_part<RB>(p, op->oldreg))] = vi[p];
             __syncthreads();
 #pragma unroll
-            for (int p = 0; p < R; p++) {
-                vi[p] = lds[swz(nl[p])];
-                li[p] = nl[p];
-            }
+            for (int p = 0; p < R; p++) vi[p] = lds[swz(nlt | reg_part<RB>(p, op->newreg))];
+            lt = nlt;
             continue;
         }
         const uint64_t eo = op->eo, ez = op->ez;
         if ((base & eo) != eo || (base & ez)) continue;  // control outside the tile not met
-        const uint32_t lc = op->lc, lz = op->lz;
-        const int ak = op->ak;
-        const double *c = op->c;
-        if (type == TOP_PAIR) {
-            const int slot = op->slot;
-            switch (ak) {
-            case AK_H: pair_slot<R, RB, AK_H>(slot, vr, vi, li, c, lc, lz); break;
-            case AK_X: pair_slot<R, RB, AK_X>(slot, vr, vi, li, c, lc, lz); break;
-            case AK_Y: pair_slot<R, RB, AK_Y>(slot, vr, vi, li, c, lc, lz); break;
-            case AK_RX: pair_slot<R, RB, AK_RX>(slot, vr, vi, li, c, lc, lz); break;
-            case AK_RY: pair_slot<R, RB, AK_RY>(slot, vr, vi, li, c, lc, lz); break;
-            case AK_U: pair_slot<R, RB, AK_U>(slot, vr, vi, li, c, lc, lz); break;
-            case AK_MAT: pair_slot<R, RB, AK_MAT>(slot, vr, vi, li, c, lc, lz); break;
-            default: break;
-            }
-        } else {
-            const int dbit = op->dbit;
-            const int ebitset = (base & op->ebit) ? 1 : 0;
-            switch (ak) {
-            case AK_NEG: diag_all<R, AK_NEG>(vr, vi, li, c, dbit, ebitset, lc, lz); break;
-            case AK_PHASE: diag_all<R, AK_PHASE>(vr, vi, li, c, dbit, ebitset, lc, lz); break;
-            case AK_RZ: diag_all<R, AK_RZ>(vr, vi, li, c, dbit, ebitset, lc, lz); break;
-            default: break;
+        const uint32_t mlo = op->mlo, mlz = op->mlz;
+        if ((lt & mlo) == mlo && !(lt & mlz)) {
+            const uint32_t mro = op->mro, mrz = op->mrz;
+            const int ak = op->ak;
+            double k[8];
+#pragma unroll
+            for (int j = 0; j < 8; j++) k[j] = op->c[j];
+            if (type == TOP_PAIR) {
+                const int slot = op->slot;
+                switch (ak) {
+                case AK_H: pair_slot<R, RB, AK_H>(slot, vr, vi, k, mro, mrz); break;
+                case AK_X: pair_slot<R, RB, AK_X>(slot, vr, vi, k, mro, mrz); break;
+                case AK_Y: pair_slot<R, RB, AK_Y>(slot, vr, vi, k, mro, mrz); break;
+                case AK_RX: pair_slot<R, RB, AK_RX>(slot, vr, vi, k, mro, mrz); break;
+                case AK_RY: pair_slot<R, RB, AK_RY>(slot, vr, vi, k, mro, mrz); break;
+                case AK_U: pair_slot<R, RB, AK_U>(slot, vr, vi, k, mro, mrz); break;
+                case AK_MAT: pair_slot<R, RB, AK_MAT>(slot, vr, vi, k, mro, mrz); break;
+                default: break;
+                }
+            } else {
+                int dsel = op->dsel;
+                const int dpos = op->dpos;
+                bool lanebit;
+                if (dsel == 2) {  // target outside the tile: the same bit for all
+                    lanebit = (base & op->ebit) != 0;
+                    dsel = 1;
+                } else {
+                    lanebit = ((lt >> dpos) & 1u) != 0;
+                }
+                switch (ak) {
+                case AK_NEG: diag_all<R, RB, AK_NEG>(vr, vi, k, dsel, dpos, lanebit, mro, mrz); break;
+                case AK_PHASE: diag_all<R, RB, AK_PHASE>(vr, vi, k, dsel, dpos, lanebit, mro, mrz); break;
+                case AK_RZ: diag_all<R, RB, AK_RZ>(vr, vi, k, dsel, dpos, lanebit, mro, mrz); break;
+                default: break;
+                }
             }
         }
     }

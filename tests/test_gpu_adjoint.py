@@ -1,0 +1,170 @@
+"""Adjoint gradients on the GPU against gradients computed by the CPU oracle.
+
+Parameter shift (E(theta + pi/2) - E(theta - pi/2)) / 2 is exact for RX, RY,
+RZ and P, so it pins the adjoint gradient to rounding (bar: 1e-10 max abs).
+Controlled rotations have no two-term shift rule; they are checked against
+central finite differences of the oracle energy (h = 1e-5, bar 1e-8).
+"""
+import math
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def oracle_energy(orc, n, ops, hx=1.0):
+    s = orc.OracleState(n)
+    orc.execute(s, ops)
+    return s.ising_energy(hx)
+
+
+def oracle_param_shift(orc, n, ops, param_ops, hx=1.0):
+    g = []
+    for i in param_ops:
+        plus = [dict(o) for o in ops]
+        minus = [dict(o) for o in ops]
+        plus[i]["theta"] += math.pi / 2
+        minus[i]["theta"] -= math.pi / 2
+        g.append(0.5 * (oracle_energy(orc, n, plus, hx) - oracle_energy(orc, n, minus, hx)))
+    return np.array(g)
+
+
+@pytest.fixture(scope="module")
+def spz():
+    import spinoza_amd
+    return spinoza_amd
+
+
+@pytest.mark.parametrize("n,layers", [(6, 2), (11, 3)])
+def test_adjoint_vs_oracle_param_shift(spz, orc, n, layers):
+    print(f"case adjoint hea n={n} layers={layers}", flush=True)
+    from spinoza_amd.autograd import adjoint_gradient
+    from spinoza_amd.workloads import hea
+    ops, thetas, where = hea(n, layers, seed=40 + n)
+    s = spz.State(n)
+    e, g, _ = adjoint_gradient(s, ops, 1.0)
+    assert abs(e - oracle_energy(orc, n, ops)) < 1e-12
+    ref = oracle_param_shift(orc, n, ops, where)
+    got = np.array([g[i] for i in where])
+    assert np.max(np.abs(got - ref)) < 1e-10, np.max(np.abs(got - ref))
+    # non-parameter ops get zero
+    others = [i for i in range(len(ops)) if i not in set(where)]
+    assert np.all(g[others] == 0.0)
+
+
+def test_adjoint_16q_vs_oracle(spz, orc):
+    print("case adjoint 16q", flush=True)
+    from spinoza_amd.autograd import adjoint_gradient
+    from spinoza_amd.workloads import hea
+    orc.set_threads(16)
+    n, layers = 16, 3
+    ops, thetas, where = hea(n, layers, seed=4)
+    s = spz.State(n)
+    e, g, _ = adjoint_gradient(s, ops, 1.0, ckpt_every=len(ops) // 3)
+    sub = where[::5]
+    ref = oracle_param_shift(orc, n, ops, sub)
+    got = np.array([g[i] for i in sub])
+    assert np.max(np.abs(got - ref)) < 1e-10, np.max(np.abs(got - ref))
+
+
+def test_adjoint_mixed_gates_vs_finite_differences(spz, orc):
+    print("case adjoint mixed", flush=True)
+    from spinoza_amd import _native as N
+    from spinoza_amd.autograd import adjoint_gradient
+    from spinoza_amd.workloads import op
+    n = 7
+    rng = np.random.default_rng(3)
+    ops = []
+    for layer in range(3):
+        for q in range(n):
+            ops.append(op(N.H, q))
+            ops.append(op([N.RX, N.RY, N.RZ, N.P][(q + layer) % 4], q, theta=rng.uniform(0, 6.28)))
+        ops.append(op(N.RX, 3, theta=0.4, ones=1 << 1))            # CRX
+        ops.append(op(N.RY, 0, theta=1.3, ones=1 << 5))            # CRY
+        ops.append(op(N.RZ, 6, theta=-0.7, ones=(1 << 2) | (1 << 4)))  # CCRZ
+        ops.append(op(N.P, 2, theta=0.9, ones=1 << 6))             # CP
+        ops.append(op(N.RY, 4, theta=0.2, zeros=1 << 0))           # 0-controlled RY
+        ops.append(op(N.U, 5, theta=0.3, phi=0.8, lam=-0.5))
+        ops.append(op(N.SWAP, 1, t1=5))
+        ops.append(op(N.X, 3, ones=1 << 2))
+        ops.append(op(N.Y, 1))
+        ops.append(op(N.Z, 4))
+    s = spz.State(n)
+    e, g, drift = adjoint_gradient(s, ops, 0.7, ckpt_every=9)
+    assert drift < 1e-13
+    assert abs(e - oracle_energy(orc, n, ops, 0.7)) < 1e-12
+    h = 1e-5
+    for i, o in enumerate(ops):
+        if o["kind"] not in (N.RX, N.RY, N.RZ, N.P):
+            assert g[i] == 0.0
+            continue
+        plus = [dict(x) for x in ops]
+        minus = [dict(x) for x in ops]
+        plus[i]["theta"] += h
+        minus[i]["theta"] -= h
+        fd = (oracle_energy(orc, n, plus, 0.7) - oracle_energy(orc, n, minus, 0.7)) / (2 * h)
+        assert abs(g[i] - fd) < 1e-8, (i, o, g[i], fd)
+
+
+def test_adjoint_checkpoints_do_not_change_gradient(spz):
+    print("case adjoint checkpoints", flush=True)
+    from spinoza_amd.autograd import adjoint_gradient
+    from spinoza_amd.workloads import hea
+    n = 14
+    ops, _, where = hea(n, 6, seed=9)
+    a = spz.State(n)
+    e0, g0, _ = adjoint_gradient(a, ops, 1.0, 0)
+    b = spz.State(n)
+    e1, g1, drift = adjoint_gradient(b, ops, 1.0, 40)
+    assert e0 == e1
+    assert drift < 1e-12
+    assert np.max(np.abs(g0 - g1)) < 1e-12
+    # the state is left holding the forward result
+    assert abs(spz.ising_energy(b, 1.0) - e1) < 1e-12
+
+
+def test_gpu_param_shift_matches_adjoint(spz):
+    print("case gpu param shift", flush=True)
+    from spinoza_amd.autograd import adjoint_gradient, param_shift_gradient
+    from spinoza_amd.workloads import hea
+    n = 13
+    ops, _, where = hea(n, 4, seed=2)
+    s = spz.State(n)
+    _, g, _ = adjoint_gradient(s, ops, 1.0)
+    ps = param_shift_gradient(n, ops, where, 1.0, subset=range(0, len(where), 7))
+    for k, v in ps.items():
+        assert abs(g[where[k]] - v) < 1e-10
+
+
+def test_torch_autograd_function(spz):
+    print("case torch autograd", flush=True)
+    import torch
+    from spinoza_amd.autograd import IsingEnergy, adjoint_gradient
+    from spinoza_amd.workloads import hea
+    n, layers = 9, 3
+
+    def builder(vals):
+        ops, _, where = hea(n, layers, thetas=vals)
+        return ops, where
+
+    th0 = hea(n, layers, seed=5)[1]
+    th = torch.tensor(th0, dtype=torch.float64, requires_grad=True)
+    e = IsingEnergy.apply(th, builder, n, 1.0)
+    e.backward()
+    ops, where = builder(th0)
+    e2, g, _ = adjoint_gradient(spz.State(n), ops, 1.0)
+    assert abs(e.item() - e2) < 1e-13
+    assert np.max(np.abs(th.grad.numpy() - np.array([g[i] for i in where]))) < 1e-13
+
+
+def test_adjoint_fp32_storage(spz):
+    print("case adjoint fp32", flush=True)
+    from spinoza_amd.autograd import adjoint_gradient
+    from spinoza_amd.workloads import hea
+    n = 12
+    ops, _, where = hea(n, 3, seed=8)
+    e64, g64, _ = adjoint_gradient(spz.State(n), ops, 1.0)
+    e32, g32, _ = adjoint_gradient(spz.State(n, "f32"), ops, 1.0)
+    assert abs(e64 - e32) < 1e-4
+    assert np.max(np.abs(g64 - g32)) < 1e-4
