@@ -42,6 +42,11 @@ def parse():
     p.add_argument("--no-strides", action="store_true", help="skip the per-stride sweep")
     p.add_argument("--cpu-threads", type=int, default=0)
     p.add_argument("--replica-layers", type=int, default=4)
+    p.add_argument("--no-adjoint", action="store_true", help="skip the 24q adjoint leg")
+    p.add_argument("--adjoint-qubits", type=int, default=24)
+    p.add_argument("--adjoint-layers", type=int, default=10)
+    p.add_argument("--ps-subset", type=int, default=24,
+                   help="parameters timed with GPU parameter shift (extrapolated to all)")
     return p.parse_args()
 
 
@@ -87,6 +92,44 @@ def per_stride(spz, N, n):
     N.check(N.lib().spz_profile_enable(0))
     del s
     return out
+
+
+def adjoint_leg(spz, N, n, layers, ps_subset, seed=4):
+    """24-qubit hardware-efficient ansatz (RY, RZ on every qubit + CX ladder,
+    10 layers, 480 parameters), Ising energy with hx = 1: adjoint gradient
+    timed over a few repetitions, GPU parameter shift timed on a subset of
+    parameters and extrapolated, max |adjoint - parameter shift| on that
+    subset."""
+    import numpy as np
+    from spinoza_amd.autograd import adjoint_gradient, param_shift_gradient
+    from spinoza_amd.workloads import hea
+    ops, thetas, where = hea(n, layers, seed=seed)
+    s = spz.State(n)
+    adjoint_gradient(s, ops, 1.0)  # warm
+    N.check(N.lib().spz_sync())
+    reps = 3
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        s.set_basis(0)
+        e, g, drift = adjoint_gradient(s, ops, 1.0, ckpt_every=len(ops) // 5)
+    N.check(N.lib().spz_sync())
+    t_adj = (time.perf_counter() - t0) / reps
+    sub = list(range(0, len(where), max(1, len(where) // ps_subset)))[:ps_subset]
+    param_shift_gradient(n, ops, where, 1.0, subset=sub[:1])  # warm
+    t0 = time.perf_counter()
+    ps = param_shift_gradient(n, ops, where, 1.0, subset=sub)
+    N.check(N.lib().spz_sync())
+    t_ps_per_param = (time.perf_counter() - t0) / len(sub)
+    err = max(abs(g[where[k]] - v) for k, v in ps.items())
+    del s
+    return {"qubits": n, "layers": layers, "params": len(where), "ops": len(ops),
+            "energy": e, "adjoint_s": round(t_adj, 4),
+            "grads_per_s": round(len(where) / t_adj, 1),
+            "param_shift_s_per_param": round(t_ps_per_param, 5),
+            "param_shift_s_full_extrapolated": round(t_ps_per_param * len(where), 3),
+            "speedup_vs_param_shift": round(t_ps_per_param * len(where) / t_adj, 1),
+            "param_shift_subset": len(sub), "max_abs_grad_err_vs_param_shift": err,
+            "checkpoint_drift": drift}
 
 
 def cpu_baseline(ops, n, threads, budget_s=20.0):
@@ -189,6 +232,10 @@ def main():
         "effective_gb_s": round(alg_bytes_circuit * a.steps / (t1 - t0) / 1e9, 1),
     }
     del s
+
+    if not a.no_adjoint:
+        log(f"adjoint leg: {a.adjoint_qubits}q, {a.adjoint_layers} layers")
+        result["adjoint"] = adjoint_leg(spz, N, a.adjoint_qubits, a.adjoint_layers, a.ps_subset)
 
     if not a.no_strides:
         log("per-stride sweep")

@@ -156,8 +156,11 @@ typedef struct spz_tracker {
 enum spz_exec_mode {
     SPZ_EXEC_AUTO = 0,      /* fused tile sweeps where they pay off         */
     SPZ_EXEC_PLAIN = 1,     /* one kernel launch per gate, in program order */
-    SPZ_EXEC_STRICT = 2     /* fused sweeps that keep program order exactly
+    SPZ_EXEC_STRICT = 2,    /* fused sweeps that keep program order exactly
                                (results bit-identical to PLAIN)             */
+    SPZ_EXEC_DENSE5 = 3     /* runs of gates on <= 5 qubits fused into dense
+                               32x32 blocks applied with MFMA (FP64 or FP32
+                               to match storage); needs n >= 12             */
 };
 
 typedef struct spz_exec_opts {
@@ -208,6 +211,43 @@ int spz_sample(spz_state s, const double *uniforms, uint64_t k, uint64_t *out);
  * the largest |psi_uncomputed - psi_checkpoint| seen. */
 int spz_adjoint_grad(spz_state s, const spz_op *ops, uint64_t n_ops, double hx,
                      int ckpt_every, double *grad, double *energy, double *drift);
+
+/* ---- batched differentiation (no counterpart in the reference) ---------- */
+typedef struct spz_batch_circuit {
+    int32_t n;              /* qubits (1..30); every circuit starts in |0...0> */
+    int32_t n_ops;          /* ops of this circuit                          */
+    uint64_t op_offset;     /* index of its first op in the shared op array */
+    uint64_t grad_offset;   /* index of its first slot in the gradient array */
+} spz_batch_circuit;
+
+typedef struct spz_batch_opts {
+    int32_t wg_max_qubits;  /* circuits with n <= this run one per workgroup in
+                               the persistent kernel; 0 = autotuned value
+                               (or 14 before spz_batch_autotune ran)        */
+    int32_t use_graph;      /* capture the persistent launch in a hipGraph  */
+    int32_t reserved[6];
+} spz_batch_opts;
+
+typedef struct spz_batch_stats {
+    double total_ms;        /* wall time of the whole batch                 */
+    double p50_ms, p99_ms;  /* per-circuit processing time                  */
+    uint64_t n_small, n_large;
+    int32_t wg_max_qubits;
+    int32_t reserved;
+} spz_batch_stats;
+
+/* Energy and adjoint gradient (as spz_adjoint_grad, no checkpoints) of every
+ * circuit: energies[i], grads[grad_offset + k] for op k of circuit i.  Ops
+ * may be single-target gates H..U with up to 7 controls.  opts and stats may
+ * be NULL. */
+int spz_batch_adjoint(const spz_batch_circuit *circuits, uint64_t count, const spz_op *ops,
+                      double hx, double *energies, double *grads,
+                      const spz_batch_opts *opts, spz_batch_stats *stats);
+/* Measure, per qubit count present in the batch, the persistent workgroup
+ * kernel against the single-circuit path and keep the switch point for later
+ * spz_batch_adjoint calls (also returned in *wg_max_qubits). */
+int spz_batch_autotune(const spz_batch_circuit *circuits, uint64_t count, const spz_op *ops,
+                       int *wg_max_qubits);
 
 /* ---- kernel timing ------------------------------------------------------ */
 enum spz_kernel_class {

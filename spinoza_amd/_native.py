@@ -17,7 +17,7 @@ F64, F32 = 0, 1
 # gate kinds (enum spz_gate_kind)
 H, X, Y, Z, P, RX, RY, RZ, U, SWAP, BITFLIP, M, UNITARY = range(13)
 # exec modes
-EXEC_AUTO, EXEC_PLAIN, EXEC_STRICT = 0, 1, 2
+EXEC_AUTO, EXEC_PLAIN, EXEC_STRICT, EXEC_DENSE5 = 0, 1, 2, 3
 OPF_CLASSICAL = 1
 # kernel classes
 K_PAIR, K_DIAG, K_TILE, K_REDUCE, K_DENSE, K_ADJOINT, K_OTHER = range(7)
@@ -56,6 +56,23 @@ class Tracker_t(ctypes.Structure):
 class ExecOpts_t(ctypes.Structure):
     _fields_ = [("mode", ctypes.c_int32), ("tile_qubits", ctypes.c_int32),
                 ("reserved", ctypes.c_int32 * 6)]
+
+
+class BatchCircuit_t(ctypes.Structure):
+    _fields_ = [("n", ctypes.c_int32), ("n_ops", ctypes.c_int32),
+                ("op_offset", ctypes.c_uint64), ("grad_offset", ctypes.c_uint64)]
+
+
+class BatchOpts_t(ctypes.Structure):
+    _fields_ = [("wg_max_qubits", ctypes.c_int32), ("use_graph", ctypes.c_int32),
+                ("reserved", ctypes.c_int32 * 6)]
+
+
+class BatchStats_t(ctypes.Structure):
+    _fields_ = [("total_ms", ctypes.c_double), ("p50_ms", ctypes.c_double),
+                ("p99_ms", ctypes.c_double), ("n_small", ctypes.c_uint64),
+                ("n_large", ctypes.c_uint64), ("wg_max_qubits", ctypes.c_int32),
+                ("reserved", ctypes.c_int32)]
 
 
 # name -> (restype, argtypes) for every exported entry point
@@ -99,6 +116,11 @@ SIGNATURES = {
     "spz_inner": (_i, [_vp, _vp, _dp, _dp]),
     "spz_sample": (_i, [_vp, _dp, _u64, _u64p]),
     "spz_adjoint_grad": (_i, [_vp, ctypes.POINTER(Op_t), _u64, _d, _i, _dp, _dp, _dp]),
+    "spz_batch_adjoint": (_i, [ctypes.POINTER(BatchCircuit_t), _u64, ctypes.POINTER(Op_t), _d,
+                               _dp, _dp, ctypes.POINTER(BatchOpts_t),
+                               ctypes.POINTER(BatchStats_t)]),
+    "spz_batch_autotune": (_i, [ctypes.POINTER(BatchCircuit_t), _u64, ctypes.POINTER(Op_t),
+                                ctypes.POINTER(_i)]),
     "spz_profile_enable": (_i, [_i]),
     "spz_profile_reset": (_i, []),
     "spz_profile_get": (_i, [_i, _u64p, _dp, _dp]),

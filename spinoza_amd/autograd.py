@@ -99,3 +99,51 @@ class IsingEnergy:
 
             cls._fn = _F
         return cls._fn.apply(thetas, builder, n, hx, ckpt_every, precision)
+
+
+def batch_adjoint(circuits, hx=1.0, wg_max_qubits=0, use_graph=True):
+    """Energies and adjoint gradients of many independent circuits.
+
+    `circuits` is a list of (n, ops); every circuit starts in |0...0>.
+    Returns (energies, [grad_per_op arrays], stats dict)."""
+    all_ops, table, off, goff = [], [], 0, 0
+    for n, ops in circuits:
+        table.append((n, len(ops), off, goff))
+        all_ops += ops
+        off += len(ops)
+        goff += len(ops)
+    arr, keep = encode_ops(all_ops)
+    cb = (N.BatchCircuit_t * max(len(table), 1))()
+    for i, (n, k, o, g) in enumerate(table):
+        cb[i].n, cb[i].n_ops, cb[i].op_offset, cb[i].grad_offset = n, k, o, g
+    energies = np.zeros(max(len(table), 1))
+    grads = np.zeros(max(goff, 1))
+    opts = N.BatchOpts_t()
+    opts.wg_max_qubits = int(wg_max_qubits)
+    opts.use_graph = int(bool(use_graph))
+    st = N.BatchStats_t()
+    N.check(N.lib().spz_batch_adjoint(cb, len(table), arr, float(hx), N.dptr(energies),
+                                      N.dptr(grads), ctypes.byref(opts), ctypes.byref(st)))
+    del keep
+    out = [grads[g:g + k].copy() for (n, k, o, g) in table]
+    stats = {"total_ms": st.total_ms, "p50_ms": st.p50_ms, "p99_ms": st.p99_ms,
+             "n_small": st.n_small, "n_large": st.n_large, "wg_max_qubits": st.wg_max_qubits}
+    return energies[:len(table)], out, stats
+
+
+def batch_autotune(circuits):
+    """Measure and keep the qubit count up to which circuits run one per
+    workgroup (see spz_batch_autotune)."""
+    all_ops, table, off = [], [], 0
+    for n, ops in circuits:
+        table.append((n, len(ops), off))
+        all_ops += ops
+        off += len(ops)
+    arr, keep = encode_ops(all_ops)
+    cb = (N.BatchCircuit_t * max(len(table), 1))()
+    for i, (n, k, o) in enumerate(table):
+        cb[i].n, cb[i].n_ops, cb[i].op_offset, cb[i].grad_offset = n, k, o, o
+    out = ctypes.c_int()
+    N.check(N.lib().spz_batch_autotune(cb, len(table), arr, ctypes.byref(out)))
+    del keep
+    return out.value

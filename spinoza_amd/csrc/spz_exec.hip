@@ -456,6 +456,137 @@ int run_fused(spz_state_s *s, const std::vector<SG> &run, int K, bool strict)
     return SPZ_OK;
 }
 
+// 2x2 matrix (row-major, interleaved re/im) of a lowered gate.
+void gate_matrix2(const Coef &k, double m[8])
+{
+    for (int i = 0; i < 8; i++) m[i] = 0.0;
+    const double *c = k.c;
+    switch (k.ak) {
+    case AK_H: m[0] = c[0]; m[2] = c[0]; m[4] = c[0]; m[6] = -c[0]; break;
+    case AK_X: m[2] = 1.0; m[4] = 1.0; break;
+    case AK_Y: m[3] = -1.0; m[5] = 1.0; break;
+    case AK_RX: m[0] = c[0]; m[3] = c[1]; m[5] = c[1]; m[6] = c[0]; break;
+    case AK_RY: m[0] = c[1]; m[2] = -c[0]; m[4] = c[0]; m[6] = c[1]; break;
+    case AK_U:
+    case AK_MAT: for (int i = 0; i < 8; i++) m[i] = c[i]; break;
+    case AK_NEG: m[0] = 1.0; m[6] = -1.0; break;
+    case AK_PHASE: m[0] = 1.0; m[6] = c[0]; m[7] = c[1]; break;
+    case AK_RZ: m[0] = c[0]; m[1] = c[1]; m[6] = c[2]; m[7] = c[3]; break;
+    default: break;
+    }
+}
+
+// Device buffer for block fragments (grown as needed, reused across calls).
+void *g_dfrag = nullptr;
+size_t g_dfrag_cap = 0;
+
+// Fuse runs of gates acting on at most 5 qubits into dense 32x32 blocks.
+int run_dense5(spz_state_s *s, const std::vector<SG> &run)
+{
+    struct Blk {
+        int q[5];
+        std::vector<int> gates;
+        int single;
+    };
+    std::vector<Blk> blocks;
+    uint64_t Q = 0;
+    std::vector<int> cur;
+    const uint64_t all = s->n >= 64 ? ~0ull : ((1ull << s->n) - 1);
+    auto flush = [&]() {
+        if (cur.empty()) return;
+        Blk b;
+        b.single = -1;
+        if (cur.size() == 1) {
+            b.single = cur[0];
+        } else {
+            int j = 0;
+            for (int q = 0; q < s->n && j < 5; q++)
+                if ((Q >> q) & 1) b.q[j++] = q;
+            for (int q = 0; q < s->n && j < 5; q++)
+                if (!((Q >> q) & 1)) { b.q[j++] = q; Q |= 1ull << q; }
+            b.gates = cur;
+        }
+        blocks.push_back(b);
+        cur.clear();
+        Q = 0;
+    };
+    for (size_t i = 0; i < run.size(); i++) {
+        const uint64_t qs = (run[i].nd | run[i].dm) & all;
+        if (__builtin_popcountll(qs) > 5) {
+            flush();
+            Blk b;
+            b.single = (int)i;
+            blocks.push_back(b);
+            continue;
+        }
+        if (__builtin_popcountll(Q | qs) > 5) flush();
+        Q |= qs;
+        cur.push_back((int)i);
+    }
+    flush();
+    // block matrices: U = G_last ... G_first, embedded in the 5-qubit block
+    std::vector<double> f64;
+    std::vector<float> f32;
+    std::vector<double> U(2 * 32 * 32);
+    for (const Blk &b : blocks) {
+        if (b.single >= 0) continue;
+        for (int i = 0; i < 32 * 32; i++) { U[2 * i] = (i / 32 == i % 32) ? 1.0 : 0.0; U[2 * i + 1] = 0.0; }
+        for (int gi : b.gates) {
+            const SG &g = run[gi];
+            double m[8];
+            gate_matrix2(g.k, m);
+            int tj = -1;
+            uint32_t need1 = 0, need0 = 0;
+            for (int j = 0; j < 5; j++) {
+                if (b.q[j] == g.target) tj = j;
+                if ((g.ones >> b.q[j]) & 1) need1 |= 1u << j;
+                if ((g.zeros >> b.q[j]) & 1) need0 |= 1u << j;
+            }
+            for (uint32_t a0 = 0; a0 < 32; a0++) {
+                if ((a0 >> tj) & 1) continue;
+                if ((a0 & need1) != need1 || (a0 & need0)) continue;
+                const uint32_t a1 = a0 | (1u << tj);
+                for (int col = 0; col < 32; col++) {
+                    const double xr = U[2 * (a0 * 32 + col)], xi = U[2 * (a0 * 32 + col) + 1];
+                    const double yr = U[2 * (a1 * 32 + col)], yi = U[2 * (a1 * 32 + col) + 1];
+                    U[2 * (a0 * 32 + col)] = m[0] * xr - m[1] * xi + m[2] * yr - m[3] * yi;
+                    U[2 * (a0 * 32 + col) + 1] = m[0] * xi + m[1] * xr + m[2] * yi + m[3] * yr;
+                    U[2 * (a1 * 32 + col)] = m[4] * xr - m[5] * xi + m[6] * yr - m[7] * yi;
+                    U[2 * (a1 * 32 + col) + 1] = m[4] * xi + m[5] * xr + m[6] * yi + m[7] * yr;
+                }
+            }
+        }
+        block5_fragments(U.data(), s->prec, f64, f32);
+    }
+    const size_t bytes = s->prec == SPZ_F64 ? f64.size() * sizeof(double) : f32.size() * sizeof(float);
+    if (bytes) {
+        SPZ_HIP(hipStreamSynchronize(rt().stream));  // the previous fragments may still be in use
+        if (bytes > g_dfrag_cap) {
+            if (g_dfrag) SPZ_HIP(hipFree(g_dfrag));
+            g_dfrag = nullptr;
+            if (hipMalloc(&g_dfrag, bytes) != hipSuccess)
+                return fail(SPZ_ERR_OOM, "block fragment allocation failed");
+            g_dfrag_cap = bytes;
+        }
+        const void *src = s->prec == SPZ_F64 ? (const void *)f64.data() : (const void *)f32.data();
+        SPZ_HIP(hipMemcpyAsync(g_dfrag, src, bytes, hipMemcpyHostToDevice, rt().stream));
+    }
+    size_t bi = 0;
+    const size_t esz = s->prec == SPZ_F64 ? sizeof(double) : sizeof(float);
+    for (const Blk &b : blocks) {
+        if (b.single >= 0) {
+            const SG &g = run[b.single];
+            if (g.diag) SPZ_TRY(launch_diag(s, g.k, g.target, g.ones, g.zeros));
+            else SPZ_TRY(launch_pair(s, g.k, g.target, g.ones, g.zeros));
+            continue;
+        }
+        SPZ_TRY(launch_block5(s, b.q, (const char *)g_dfrag + bi * 4096 * esz));
+        bi++;
+    }
+    if (bytes) SPZ_HIP(hipStreamSynchronize(rt().stream));
+    return SPZ_OK;
+}
+
 }  // namespace
 
 int execute_plain(spz_state_s *s, const spz_op *ops, uint64_t n_ops, spz_tracker *trk,
@@ -476,8 +607,10 @@ int execute_ops(spz_state_s *s, const spz_op *ops, uint64_t n_ops, spz_tracker *
 {
     int mode = opts ? opts->mode : SPZ_EXEC_AUTO;
     if (mode == SPZ_EXEC_PLAIN) return execute_plain(s, ops, n_ops, trk, outcomes);
-    if (mode != SPZ_EXEC_AUTO && mode != SPZ_EXEC_STRICT)
+    if (mode != SPZ_EXEC_AUTO && mode != SPZ_EXEC_STRICT && mode != SPZ_EXEC_DENSE5)
         return fail(SPZ_ERR_ARG, "unknown execution mode");
+    const bool dense5 = mode == SPZ_EXEC_DENSE5;
+    if (dense5 && s->n < 12) return execute_plain(s, ops, n_ops, trk, outcomes);
     int K = (opts && opts->tile_qubits) ? opts->tile_qubits : env_int("SPZ_TILE_K", 12);
     if (K < 10 || K > 13) return fail(SPZ_ERR_ARG, "tile_qubits must be in [10, 13]");
     if (s->n < K + 1) K = s->n - 1;
@@ -508,7 +641,7 @@ int execute_ops(spz_state_s *s, const spz_op *ops, uint64_t n_ops, spz_tracker *
             }
             run.push_back(g);
         }
-        if (!run.empty()) SPZ_TRY(run_fused(s, run, K, strict));
+        if (!run.empty()) SPZ_TRY(dense5 ? run_dense5(s, run) : run_fused(s, run, K, strict));
         i = j;
     }
     return SPZ_OK;

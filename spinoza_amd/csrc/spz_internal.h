@@ -267,6 +267,10 @@ int apply_gate(spz_state_s *s, int kind, double th, double ph, double la, int ta
                int t1, uint64_t ones, uint64_t zeros);
 int measure(spz_state_s *s, int target, int reset, int forced, double u, uint8_t *out);
 
+void block5_fragments(const double *U, int prec, std::vector<double> &out64,
+                      std::vector<float> &out32);
+int launch_block5(spz_state_s *s, const int *q, const void *dfrag);
+
 int execute_ops(spz_state_s *s, const spz_op *ops, uint64_t n_ops, spz_tracker *trk,
                 const spz_exec_opts *opts, uint8_t *outcomes);
 

@@ -168,3 +168,24 @@ def test_adjoint_fp32_storage(spz):
     e32, g32, _ = adjoint_gradient(spz.State(n, "f32"), ops, 1.0)
     assert abs(e64 - e32) < 1e-4
     assert np.max(np.abs(g64 - g32)) < 1e-4
+
+
+def test_batched_adjoint_matches_single(spz):
+    print("case batched adjoint", flush=True)
+    from spinoza_amd.autograd import adjoint_gradient, batch_adjoint
+    from spinoza_amd.rng import SplitMix64
+    from spinoza_amd.workloads import hea
+    rng = SplitMix64(55)
+    circuits = []
+    for i in range(40):
+        n = 4 + rng.next_u64() % 9          # 4..12
+        depth = 1 + rng.next_u64() % 4      # 1..4
+        ops, _, _ = hea(n, depth, seed=1000 + i)
+        circuits.append((n, ops))
+    for wg_max, graph in ((16, True), (8, False)):
+        energies, grads, stats = batch_adjoint(circuits, 0.8, wg_max_qubits=wg_max, use_graph=graph)
+        assert stats["n_small"] + stats["n_large"] == len(circuits)
+        for (n, ops), e, g in zip(circuits, energies, grads):
+            e1, g1, _ = adjoint_gradient(spz.State(n), ops, 0.8)
+            assert abs(e - e1) < 1e-12
+            assert np.max(np.abs(g - g1)) < 1e-12

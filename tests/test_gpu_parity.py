@@ -303,3 +303,36 @@ def test_full_size_properties(spz):
         spz.apply(spz.Gate.X, s, q)
     assert s[(1 << n) - 1] == (1.0, 0.0) and s[0] == (0.0, 0.0)
     del s
+
+
+@pytest.mark.parametrize("n", [12, 15])
+def test_dense5_mfma_blocks_vs_oracle(spz, orc, n):
+    """Gates fused into dense 5-qubit blocks and applied with FP64 MFMA."""
+    print(f"case dense5 n={n}", flush=True)
+    from spinoza_amd import _native as N
+    from spinoza_amd.executor import run_ops
+    from spinoza_amd.workloads import qft, random_circuit
+    for ops in (random_circuit(n, 5, 21 + n), qft(n)):
+        ref = orc.OracleState(n)
+        orc.execute(ref, ops)
+        s = spz.State(n)
+        run_ops(s, ops, mode=N.EXEC_DENSE5)
+        assert_match(s, ref, exact=False, what=f"dense5 n={n}")
+
+
+def test_dense5_fp32_fidelity(spz, orc):
+    print("case dense5 fp32", flush=True)
+    from spinoza_amd import _native as N
+    from spinoza_amd.executor import run_ops
+    from spinoza_amd.workloads import random_circuit
+    n = 16
+    ops = random_circuit(n, 6, 3)
+    ref = orc.OracleState(n)
+    orc.execute(ref, ops)
+    s = spz.State(n, "f32")
+    run_ops(s, ops, mode=N.EXEC_DENSE5)
+    psi = s.to_numpy()
+    nrm = s.norm()
+    fid = abs(np.vdot(ref.re + 1j * ref.im, psi)) ** 2 / nrm
+    assert 1 - fid < 1e-8, fid
+    assert abs(nrm - 1.0) < 1e-5
