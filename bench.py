@@ -45,6 +45,9 @@ def parse():
     p.add_argument("--no-adjoint", action="store_true", help="skip the 24q adjoint leg")
     p.add_argument("--adjoint-qubits", type=int, default=24)
     p.add_argument("--adjoint-layers", type=int, default=10)
+    p.add_argument("--calibrate", action="store_true",
+                   help="first apply one plain X on the top qubit: a pass that reads and "
+                        "writes every byte of the state once (counter calibration)")
     p.add_argument("--ps-subset", type=int, default=24,
                    help="parameters timed with GPU parameter shift (extrapolated to all)")
     return p.parse_args()
@@ -171,6 +174,10 @@ def main():
     log(f"workload: {n}q, {L} layers, {len(ops)} gates, mode={a.mode}")
 
     s = spz.State(n)
+    if a.calibrate:
+        spz.apply(spz.Gate.X, s, n - 1)
+        N.check(N.lib().spz_sync())
+        log(f"calibration pass: X on qubit {n - 1}, {32 * (1 << n)} bytes moved")
     for w in range(a.warmup):
         s.set_basis(0)
         run_ops(s, ops, mode=mode, encoded=enc)

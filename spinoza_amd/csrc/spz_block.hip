@@ -15,9 +15,9 @@
 // fragments from LDS and keeping the 64 A fragments of the matrix in registers,
 // writes the results back into the same LDS slots, and the tile is stored.
 //
-// MFMA lane layout (16x16x4, f64 and f32, checked on gfx950): lane i supplies
-// A[i % 16][i / 16] and B[i / 16][i % 16]; its 4 accumulator slots j hold
-// D[4 (i / 16) + j][i % 16].
+// MFMA lane layout (16x16x4, measured on gfx950): lane i supplies A[i % 16][i / 16]
+// and B[i / 16][i % 16] for both types; accumulator slot j of lane i holds
+// D[4 (i / 16) + j][i % 16] for f32 but D[4 j + i / 16][i % 16] for f64.
 #include <string.h>
 
 #include <vector>
@@ -42,8 +42,15 @@ typedef double v4d __attribute__((ext_vector_type(4)));
 typedef float v4f __attribute__((ext_vector_type(4)));
 
 template <typename T> struct Acc;
-template <> struct Acc<double> { using type = v4d; };
-template <> struct Acc<float> { using type = v4f; };
+template <> struct Acc<double> {
+    using type = v4d;
+    // output row (within a 16-row block) of accumulator slot j in lane group kq
+    __device__ static uint32_t row(uint32_t kq, uint32_t j) { return 4u * j + kq; }
+};
+template <> struct Acc<float> {
+    using type = v4f;
+    __device__ static uint32_t row(uint32_t kq, uint32_t j) { return 4u * kq + j; }
+};
 
 __device__ __forceinline__ v4d mfma(double a, double b, v4d c)
 {
@@ -126,7 +133,7 @@ k_block5(T *__restrict__ re, T *__restrict__ im, const BlockDesc d, const T *__r
         for (int r = 0; r < 4; r++)
 #pragma unroll
             for (int j = 0; j < 4; j++) {
-                const uint32_t c = 16u * r + 4u * kq + j;
+                const uint32_t c = 16u * r + Acc<T>::row(kq, j);
                 const uint32_t l = vl | dep5(c & 31u, d.bpos);
                 if (c < 32) lre[l] = acc[r][j];
                 else lim[l] = acc[r][j];

@@ -186,13 +186,13 @@ struct Planner {
         op.type = TOP_LAYOUT;
         uint32_t used = 0;
         for (int j = 0; j < RB; j++) {
-            op.oldreg[j] = (int8_t)oldRS[j];
-            op.newreg[j] = (int8_t)RSv[j];
+            op.lay.oldreg[j] = (int8_t)oldRS[j];
+            op.lay.newreg[j] = (int8_t)RSv[j];
             used |= 1u << RSv[j];
         }
         int i = 0;
         for (int b = 0; b < K; b++)
-            if (!((used >> b) & 1)) op.newthr[i++] = (int8_t)b;
+            if (!((used >> b) & 1)) op.lay.newthr[i++] = (int8_t)b;
         prog.push_back(op);
     }
 

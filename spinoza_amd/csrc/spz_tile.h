@@ -22,6 +22,14 @@ namespace spz {
 
 enum TileOpType : int32_t { TOP_PAIR = 0, TOP_DIAG = 1, TOP_LAYOUT = 2 };
 
+struct TileLayout {
+    int8_t oldreg[8];   // local bit of register-index bit j before ...
+    int8_t newreg[8];   // ... and after the change
+    int8_t newthr[16];  // local bit of thread-index bit i after the change
+};
+
+// 128 bytes: fetched whole into scalar registers, the next one prefetched
+// while the current one executes.
 struct TileOp {
     int32_t type;     // TileOpType
     int32_t ak;       // Arith kind
@@ -34,11 +42,12 @@ struct TileOp {
     uint32_t pad;
     uint64_t eo, ez;    // controls outside S (global index bits), uniform per tile
     uint64_t ebit;      // DIAG with the target outside S: its global bit
-    double c[8];        // coefficients
-    int8_t oldreg[8];   // LAYOUT: local bit of register-index bit j before ...
-    int8_t newreg[8];   // ... and after the change
-    int8_t newthr[16];  // LAYOUT: local bit of thread-index bit i after the change
+    union {
+        double c[8];    // PAIR / DIAG coefficients
+        TileLayout lay; // LAYOUT
+    };
 };
+static_assert(sizeof(TileOp) == 128, "TileOp must stay 128 bytes");
 
 constexpr int kTileMaxK = 14;
 constexpr int kTileMaxR = 32;

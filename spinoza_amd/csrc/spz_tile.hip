@@ -181,68 +181,68 @@ k_tile(T *__restrict__ re, T *__restrict__ im, const TileDesc d, const TileOp *_
     uint32_t lt = tid;
 
     const int nops = d.nops;
+    TileOp cur = ops[0];
     for (int o = 0; o < nops; o++) {
-        const TileOp *op = ops + o;
-        const int type = op->type;
-        if (type == TOP_LAYOUT) {
+        // issue the next op's scalar loads before working on this one
+        const TileOp nxt = ops[o + 1 < nops ? o + 1 : o];
+        if (cur.type == TOP_LAYOUT) {
             uint32_t nlt = 0;
 #pragma unroll
             for (int i = 0; i < LOGW; i++)
-                if ((tid >> i) & 1u) nlt |= 1u << op->newthr[i];
+                if ((tid >> i) & 1u) nlt |= 1u << cur.lay.newthr[i];
             __syncthreads();
 #pragma unroll
-            for (int p = 0; p < R; p++) lds[swz(lt | reg_part<RB>(p, op->oldreg))] = vr[p];
+            for (int p = 0; p < R; p++) lds[swz(lt | reg_part<RB>(p, cur.lay.oldreg))] = vr[p];
             __syncthreads();
 #pragma unroll
-            for (int p = 0; p < R; p++) vr[p] = lds[swz(nlt | reg_part<RB>(p, op->newreg))];
+            for (int p = 0; p < R; p++) vr[p] = lds[swz(nlt | reg_part<RB>(p, cur.lay.newreg))];
             __syncthreads();
 #pragma unroll
-            for (int p = 0; p < R; p++) lds[swz(lt | reg_part<RB>(p, op->oldreg))] = vi[p];
+            for (int p = 0; p < R; p++) lds[swz(lt | reg_part<RB>(p, cur.lay.oldreg))] = vi[p];
             __syncthreads();
 #pragma unroll
-            for (int p = 0; p < R; p++) vi[p] = lds[swz(nlt | reg_part<RB>(p, op->newreg))];
+            for (int p = 0; p < R; p++) vi[p] = lds[swz(nlt | reg_part<RB>(p, cur.lay.newreg))];
             lt = nlt;
-            continue;
-        }
-        const uint64_t eo = op->eo, ez = op->ez;
-        if ((base & eo) != eo || (base & ez)) continue;  // control outside the tile not met
-        const uint32_t mlo = op->mlo, mlz = op->mlz;
-        if ((lt & mlo) == mlo && !(lt & mlz)) {
-            const uint32_t mro = op->mro, mrz = op->mrz;
-            const int ak = op->ak;
-            double k[8];
-#pragma unroll
-            for (int j = 0; j < 8; j++) k[j] = op->c[j];
-            if (type == TOP_PAIR) {
-                const int slot = op->slot;
-                switch (ak) {
-                case AK_H: pair_slot<R, RB, AK_H>(slot, vr, vi, k, mro, mrz); break;
-                case AK_X: pair_slot<R, RB, AK_X>(slot, vr, vi, k, mro, mrz); break;
-                case AK_Y: pair_slot<R, RB, AK_Y>(slot, vr, vi, k, mro, mrz); break;
-                case AK_RX: pair_slot<R, RB, AK_RX>(slot, vr, vi, k, mro, mrz); break;
-                case AK_RY: pair_slot<R, RB, AK_RY>(slot, vr, vi, k, mro, mrz); break;
-                case AK_U: pair_slot<R, RB, AK_U>(slot, vr, vi, k, mro, mrz); break;
-                case AK_MAT: pair_slot<R, RB, AK_MAT>(slot, vr, vi, k, mro, mrz); break;
-                default: break;
-                }
-            } else {
-                int dsel = op->dsel;
-                const int dpos = op->dpos;
-                bool lanebit;
-                if (dsel == 2) {  // target outside the tile: the same bit for all
-                    lanebit = (base & op->ebit) != 0;
-                    dsel = 1;
+        } else if ((base & cur.eo) == cur.eo && !(base & cur.ez)) {
+            // (a control outside the tile that is not met skips the op)
+            const uint32_t mlo = cur.mlo, mlz = cur.mlz;
+            const bool lane_ok = ((mlo | mlz) == 0) || ((lt & mlo) == mlo && !(lt & mlz));
+            if (lane_ok) {
+                const uint32_t mro = cur.mro, mrz = cur.mrz;
+                const int ak = cur.ak;
+                const double *k = cur.c;
+                if (cur.type == TOP_PAIR) {
+                    const int slot = cur.slot;
+                    switch (ak) {
+                    case AK_H: pair_slot<R, RB, AK_H>(slot, vr, vi, k, mro, mrz); break;
+                    case AK_X: pair_slot<R, RB, AK_X>(slot, vr, vi, k, mro, mrz); break;
+                    case AK_Y: pair_slot<R, RB, AK_Y>(slot, vr, vi, k, mro, mrz); break;
+                    case AK_RX: pair_slot<R, RB, AK_RX>(slot, vr, vi, k, mro, mrz); break;
+                    case AK_RY: pair_slot<R, RB, AK_RY>(slot, vr, vi, k, mro, mrz); break;
+                    case AK_U: pair_slot<R, RB, AK_U>(slot, vr, vi, k, mro, mrz); break;
+                    case AK_MAT: pair_slot<R, RB, AK_MAT>(slot, vr, vi, k, mro, mrz); break;
+                    default: break;
+                    }
                 } else {
-                    lanebit = ((lt >> dpos) & 1u) != 0;
-                }
-                switch (ak) {
-                case AK_NEG: diag_all<R, RB, AK_NEG>(vr, vi, k, dsel, dpos, lanebit, mro, mrz); break;
-                case AK_PHASE: diag_all<R, RB, AK_PHASE>(vr, vi, k, dsel, dpos, lanebit, mro, mrz); break;
-                case AK_RZ: diag_all<R, RB, AK_RZ>(vr, vi, k, dsel, dpos, lanebit, mro, mrz); break;
-                default: break;
+                    int dsel = cur.dsel;
+                    const int dpos = cur.dpos;
+                    bool lanebit;
+                    if (dsel == 2) {  // target outside the tile: the same bit for all
+                        lanebit = (base & cur.ebit) != 0;
+                        dsel = 1;
+                    } else {
+                        lanebit = ((lt >> dpos) & 1u) != 0;
+                    }
+                    switch (ak) {
+                    case AK_NEG: diag_all<R, RB, AK_NEG>(vr, vi, k, dsel, dpos, lanebit, mro, mrz); break;
+                    case AK_PHASE: diag_all<R, RB, AK_PHASE>(vr, vi, k, dsel, dpos, lanebit, mro, mrz); break;
+                    case AK_RZ: diag_all<R, RB, AK_RZ>(vr, vi, k, dsel, dpos, lanebit, mro, mrz); break;
+                    default: break;
+                    }
                 }
             }
         }
+        cur = nxt;
     }
 
 #pragma unroll

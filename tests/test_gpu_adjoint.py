@@ -117,7 +117,8 @@ def test_adjoint_checkpoints_do_not_change_gradient(spz):
     e0, g0, _ = adjoint_gradient(a, ops, 1.0, 0)
     b = spz.State(n)
     e1, g1, drift = adjoint_gradient(b, ops, 1.0, 40)
-    assert e0 == e1
+    # the checkpointed forward pass is fused in segments: rounding may differ
+    assert abs(e0 - e1) < 1e-14
     assert drift < 1e-12
     assert np.max(np.abs(g0 - g1)) < 1e-12
     # the state is left holding the forward result
