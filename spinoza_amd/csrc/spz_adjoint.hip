@@ -7,10 +7,13 @@
 //   and for P(theta) = diag(1, e^{i theta}), dE/dtheta_k = -2 Im <lambda_k|Pi_1~|psi_k>
 //   (G~, Pi_1~: the operator restricted to amplitudes whose controls are met).
 //
-// Reverse sweep, one fused pass per gate: read the (psi, lambda) pairs of the
-// gate, accumulate the gradient term (TwoSum-compensated FP64 per thread, a
-// halving tree per block, one more block for the block partials, written to
-// a device gradient array), apply U_k^dag to both, write both back.
+// Reverse sweep.  Consecutive ops whose targets fit in 3 qubits form a group
+// that costs one pass over psi and lambda: per element (the 8 amplitudes that
+// differ in those qubits) each op adds its gradient term (TwoSum-compensated
+// FP64 per thread, a halving tree per block, one more block over the block
+// partials, written to a device gradient array) and then applies U_k^dag to
+// both states.  Runs of 4 or more parameter-free ops are undone with the
+// fused tile executor, once on psi and once on lambda.
 #include <math.h>
 #include <string.h>
 
@@ -28,7 +31,6 @@ int execute_ops(spz_state_s *s, const spz_op *ops, uint64_t n_ops, spz_tracker *
 namespace {
 
 constexpr int kT = 256;
-constexpr int kPerThread = 8;  // pairs per thread per block
 
 enum GradKind { GK_NONE = 0, GK_X = 1, GK_Y = 2, GK_Z = 3, GK_P = 4 };
 
@@ -51,76 +53,148 @@ __device__ __forceinline__ double re_cj(double ar, double ai, double br, double 
     return ar * br + ai * bi;
 }
 
-template <typename T, int AK, int GK>
-__global__ void __launch_bounds__(kT)
-k_adj_step(T *__restrict__ pr, T *__restrict__ pi, T *__restrict__ lr, T *__restrict__ li,
-           uint64_t npairs, int target, uint64_t ones, PosList pl, Coef kc,
-           double *__restrict__ partials)
+// A reverse-sweep group: up to kGMaxOps consecutive ops (taken in reverse
+// order) whose targets lie in a set Q of at most kGMaxQ qubits.  One pass
+// visits every element (the 2^|Q| amplitudes that differ only in the Q bits,
+// with the common control bits fixed) and applies each op's inverse to both
+// psi and lambda, after adding the op's gradient term for that element.
+constexpr int kGMaxOps = 8;
+constexpr int kGMaxQ = 3;
+constexpr int kGPerThread = 4;  // elements per thread
+
+struct GroupOp {
+    int32_t ak, gk, tq, pad;  // arithmetic, gradient kind, index of the target in Q
+    uint64_t cm, ones;        // control mask and the control bits that must be 1
+    double c[8];              // inverse-gate coefficients
+};
+
+struct GroupDesc {
+    int8_t q[kGMaxQ + 1];  // the qubits of Q
+    int32_t nop;
+    uint64_t base_ones;    // common controls that must be 1 (fixed in every element)
+    GroupOp op[kGMaxOps];
+};
+
+__device__ __forceinline__ double grad_term(int gk, double la, double lb, double lc, double ld,
+                                            double a, double b, double c, double d)
 {
-    __shared__ double S[kT], C[kT];
-    const uint64_t tb = 1ull << target;
-    const uint64_t block0 = (uint64_t)blockIdx.x * kT * kPerThread;
-    double s = 0.0, c = 0.0;
-#pragma unroll 2
-    for (int e = 0; e < kPerThread; e++) {
-        const uint64_t u = block0 + (uint64_t)e * kT + threadIdx.x;
-        if (u >= npairs) break;
-        const uint64_t s0 = insert_zeros(u, pl.pos, pl.np) | ones;
-        const uint64_t s1 = s0 | tb;
-        double a = pr[s0], b = pi[s0], cc = pr[s1], d = pi[s1];
-        double la = lr[s0], lb = li[s0], lc = lr[s1], ld = li[s1];
-        if constexpr (GK != GK_NONE) {
-            double t;
-            if constexpr (GK == GK_X)  // Im <l|X|p> = Im(l0* p1) + Im(l1* p0)
-                t = im_cj(la, lb, cc, d) + im_cj(lc, ld, a, b);
-            else if constexpr (GK == GK_Y)  // Im <l|Y|p> = Re(l1* p0) - Re(l0* p1)
-                t = re_cj(lc, ld, a, b) - re_cj(la, lb, cc, d);
-            else if constexpr (GK == GK_Z)  // Im <l|Z|p> = Im(l0* p0) - Im(l1* p1)
-                t = im_cj(la, lb, a, b) - im_cj(lc, ld, cc, d);
-            else  // -2 Im <l|Pi_1|p>
-                t = -2.0 * im_cj(lc, ld, cc, d);
+    if (gk == GK_X)  // Im <l|X|p> = Im(l0* p1) + Im(l1* p0)
+        return im_cj(la, lb, c, d) + im_cj(lc, ld, a, b);
+    if (gk == GK_Y)  // Im <l|Y|p> = Re(l1* p0) - Re(l0* p1)
+        return re_cj(lc, ld, a, b) - re_cj(la, lb, c, d);
+    if (gk == GK_Z)  // Im <l|Z|p> = Im(l0* p0) - Im(l1* p1)
+        return im_cj(la, lb, a, b) - im_cj(lc, ld, c, d);
+    return -2.0 * im_cj(lc, ld, c, d);  // P: -2 Im <l|Pi_1|p>
+}
+
+// Apply op (target = Q[TQ]) to the pairs of one element, adding the gradient
+// terms of the pairs whose controls are met to (s, e).
+template <int M, int TQ>
+__device__ __forceinline__ void group_apply(const GroupOp &op, uint64_t base, const uint64_t *off,
+                                            double *ar, double *ai, double *lr, double *li,
+                                            double &s, double &e)
+{
+#pragma unroll
+    for (int x = 0; x < (1 << M); x++) {
+        if (x & (1 << TQ)) continue;
+        const int y = x | (1 << TQ);
+        if (((base | off[x]) & op.cm) != op.ones) continue;
+        if (op.gk != GK_NONE) {
+            const double t = grad_term(op.gk, lr[x], li[x], lr[y], li[y], ar[x], ai[x], ar[y], ai[y]);
             double ns, er;
             two_sum_d(s, t, ns, er);
             s = ns;
-            c = c + er;
+            e = e + er;
         }
-        pair_update(AK, kc.c, a, b, cc, d);
-        pair_update(AK, kc.c, la, lb, lc, ld);
-        pr[s0] = (T)a; pi[s0] = (T)b; pr[s1] = (T)cc; pi[s1] = (T)d;
-        lr[s0] = (T)la; li[s0] = (T)lb; lr[s1] = (T)lc; li[s1] = (T)ld;
-    }
-    if constexpr (GK != GK_NONE) {
-        const int tid = threadIdx.x;
-        S[tid] = s;
-        C[tid] = c;
-        __syncthreads();
-        for (int h = kT / 2; h >= 1; h >>= 1) {
-            if (tid < h) {
-                double ns, e;
-                two_sum_d(S[tid], S[tid + h], ns, e);
-                C[tid] = (C[tid] + C[tid + h]) + e;
-                S[tid] = ns;
-            }
-            __syncthreads();
-        }
-        if (tid == 0) {
-            partials[2 * blockIdx.x] = S[0];
-            partials[2 * blockIdx.x + 1] = C[0];
-        }
+        pair_update(op.ak, op.c, ar[x], ai[x], ar[y], ai[y]);
+        pair_update(op.ak, op.c, lr[x], li[x], lr[y], li[y]);
     }
 }
 
-// One block: reduce nb block partials into out[0].
+// partials[(j * gridDim.x + block) * 2 + {0, 1}]: compensated block sum of
+// op j's gradient terms (ops without a gradient write nothing).
+template <typename T, int M>
 __global__ void __launch_bounds__(kT)
-k_adj_finish(const double *__restrict__ partials, uint64_t nb, double *__restrict__ out)
+k_adj_group(T *__restrict__ pr, T *__restrict__ pi, T *__restrict__ qr, T *__restrict__ qi,
+            uint64_t nel, PosList pl, GroupDesc gd, double *__restrict__ partials)
+{
+    __shared__ double S[kGMaxOps][kT], C[kGMaxOps][kT];
+    const int tid = threadIdx.x;
+    const int nop = gd.nop;
+    for (int j = 0; j < nop; j++) {
+        S[j][tid] = 0.0;
+        C[j][tid] = 0.0;
+    }
+    uint64_t off[1 << M];
+#pragma unroll
+    for (int x = 0; x < (1 << M); x++) {
+        uint64_t o = 0;
+#pragma unroll
+        for (int b = 0; b < M; b++)
+            if (x & (1 << b)) o |= 1ull << gd.q[b];
+        off[x] = o;
+    }
+    const uint64_t block0 = (uint64_t)blockIdx.x * kT * kGPerThread;
+    for (int w = 0; w < kGPerThread; w++) {
+        const uint64_t u = block0 + (uint64_t)w * kT + tid;
+        if (u >= nel) break;
+        const uint64_t base = insert_zeros(u, pl.pos, pl.np) | gd.base_ones;
+        double ar[1 << M], ai[1 << M], lr[1 << M], li[1 << M];
+#pragma unroll
+        for (int x = 0; x < (1 << M); x++) {
+            const uint64_t i = base | off[x];
+            ar[x] = pr[i]; ai[x] = pi[i]; lr[x] = qr[i]; li[x] = qi[i];
+        }
+        for (int j = 0; j < nop; j++) {
+            const GroupOp &op = gd.op[j];
+            double s = S[j][tid], e = C[j][tid];
+            if (op.tq == 0) group_apply<M, 0>(op, base, off, ar, ai, lr, li, s, e);
+            if constexpr (M > 1) if (op.tq == 1) group_apply<M, 1>(op, base, off, ar, ai, lr, li, s, e);
+            if constexpr (M > 2) if (op.tq == 2) group_apply<M, 2>(op, base, off, ar, ai, lr, li, s, e);
+            S[j][tid] = s;
+            C[j][tid] = e;
+        }
+#pragma unroll
+        for (int x = 0; x < (1 << M); x++) {
+            const uint64_t i = base | off[x];
+            pr[i] = (T)ar[x]; pi[i] = (T)ai[x]; qr[i] = (T)lr[x]; qi[i] = (T)li[x];
+        }
+    }
+    __syncthreads();
+    for (int h = kT / 2; h >= 1; h >>= 1) {
+        if (tid < h)
+            for (int j = 0; j < nop; j++) {
+                double ns, e;
+                two_sum_d(S[j][tid], S[j][tid + h], ns, e);
+                C[j][tid] = (C[j][tid] + C[j][tid + h]) + e;
+                S[j][tid] = ns;
+            }
+        __syncthreads();
+    }
+    if (tid < nop && gd.op[tid].gk != GK_NONE) {
+        const uint64_t k = ((uint64_t)tid * gridDim.x + blockIdx.x) * 2;
+        partials[k] = S[tid][0];
+        partials[k + 1] = C[tid][0];
+    }
+}
+
+// Block j reduces op j's nb block partials into grad[idx.i[j]].
+struct GradIdx {
+    int64_t i[kGMaxOps];
+};
+
+__global__ void __launch_bounds__(kT)
+k_adj_finish(const double *__restrict__ partials, uint64_t nb, GradIdx idx,
+             double *__restrict__ grad)
 {
     __shared__ double S[kT], C[kT];
     const int tid = threadIdx.x;
+    const double *p = partials + (uint64_t)blockIdx.x * nb * 2;
     double s = 0.0, c = 0.0;
     for (uint64_t j = tid; j < nb; j += kT) {
         double ns, e;
-        two_sum_d(s, partials[2 * j], ns, e);
-        c = (c + partials[2 * j + 1]) + e;
+        two_sum_d(s, p[2 * j], ns, e);
+        c = (c + p[2 * j + 1]) + e;
         s = ns;
     }
     S[tid] = s;
@@ -135,7 +209,7 @@ k_adj_finish(const double *__restrict__ partials, uint64_t nb, double *__restric
         }
         __syncthreads();
     }
-    if (tid == 0) out[0] = S[0] + C[0];
+    if (tid == 0 && idx.i[blockIdx.x] >= 0) grad[idx.i[blockIdx.x]] = S[0] + C[0];
 }
 
 // lambda = H psi, H = sum_{i<n-1} Z_i Z_{i+1} + hx sum_i X_i (same operation
@@ -179,43 +253,55 @@ inline unsigned blocks_for(uint64_t units, uint64_t per_block)
     return (unsigned)(g == 0 ? 1 : g);
 }
 
-template <typename T, int AK>
-int step_gk(spz_state_s *psi, spz_state_s *lam, uint64_t npairs, int target, uint64_t ones,
-            const PosList &pl, const Coef &k, int gk, double *partials, unsigned g)
+bool has_param(int kind)
 {
-    T *pr = (T *)psi->re, *pi = (T *)psi->im, *lr = (T *)lam->re, *li = (T *)lam->im;
-    hipStream_t st = rt().stream;
-#define SPZ_ADJ(GK) hipLaunchKernelGGL((k_adj_step<T, AK, GK>), dim3(g), dim3(kT), 0, st, pr, pi, lr, li, npairs, target, ones, pl, k, partials)
-    switch (gk) {
-    case GK_NONE: SPZ_ADJ(GK_NONE); break;
-    case GK_X: SPZ_ADJ(GK_X); break;
-    case GK_Y: SPZ_ADJ(GK_Y); break;
-    case GK_Z: SPZ_ADJ(GK_Z); break;
-    case GK_P: SPZ_ADJ(GK_P); break;
-    default: return fail(SPZ_ERR_ARG, "bad gradient kind");
-    }
-#undef SPZ_ADJ
-    SPZ_HIP(hipGetLastError());
-    return SPZ_OK;
+    return kind == SPZ_RX || kind == SPZ_RY || kind == SPZ_RZ || kind == SPZ_P;
 }
 
-template <typename T>
-int step(spz_state_s *psi, spz_state_s *lam, uint64_t npairs, int target, uint64_t ones,
-         const PosList &pl, const Coef &k, int gk, double *partials, unsigned g)
+int grad_kind(int kind)
 {
-    switch (k.ak) {
-    case AK_H: return step_gk<T, AK_H>(psi, lam, npairs, target, ones, pl, k, gk, partials, g);
-    case AK_X: return step_gk<T, AK_X>(psi, lam, npairs, target, ones, pl, k, gk, partials, g);
-    case AK_Y: return step_gk<T, AK_Y>(psi, lam, npairs, target, ones, pl, k, gk, partials, g);
-    case AK_RX: return step_gk<T, AK_RX>(psi, lam, npairs, target, ones, pl, k, gk, partials, g);
-    case AK_RY: return step_gk<T, AK_RY>(psi, lam, npairs, target, ones, pl, k, gk, partials, g);
-    case AK_U: return step_gk<T, AK_U>(psi, lam, npairs, target, ones, pl, k, gk, partials, g);
-    case AK_MAT: return step_gk<T, AK_MAT>(psi, lam, npairs, target, ones, pl, k, gk, partials, g);
-    case AK_NEG: return step_gk<T, AK_NEG>(psi, lam, npairs, target, ones, pl, k, gk, partials, g);
-    case AK_PHASE: return step_gk<T, AK_PHASE>(psi, lam, npairs, target, ones, pl, k, gk, partials, g);
-    case AK_RZ: return step_gk<T, AK_RZ>(psi, lam, npairs, target, ones, pl, k, gk, partials, g);
-    default: return fail(SPZ_ERR_GATE, "adjoint: unsupported arithmetic");
+    switch (kind) {
+    case SPZ_RX: return GK_X;
+    case SPZ_RY: return GK_Y;
+    case SPZ_RZ: return GK_Z;
+    case SPZ_P: return GK_P;
+    default: return GK_NONE;
     }
+}
+
+// The inverse of one op (gates.rs:78-92): rotations and P negate theta,
+// U(theta, phi, lambda)^-1 = U(-theta, -lambda, -phi), the rest are self-inverse.
+spz_op inverse_op(const spz_op &o)
+{
+    spz_op r = o;
+    if (has_param(o.kind)) r.theta = -o.theta;
+    if (o.kind == SPZ_U) {
+        r.theta = -o.theta;
+        r.phi = -o.lambda;
+        r.lambda = -o.phi;
+    }
+    return r;
+}
+
+int launch_group(spz_state_s *psi, spz_state_s *lam, int nq, uint64_t nel, const PosList &pl,
+                 const GroupDesc &gd, double *partials, unsigned g)
+{
+    hipStream_t st = rt().stream;
+#define SPZ_GRP(T, M)                                                                        \
+    hipLaunchKernelGGL((k_adj_group<T, M>), dim3(g), dim3(kT), 0, st, (T *)psi->re,            \
+                       (T *)psi->im, (T *)lam->re, (T *)lam->im, nel, pl, gd, partials)
+    if (psi->prec == SPZ_F64) {
+        if (nq == 1) SPZ_GRP(double, 1);
+        else if (nq == 2) SPZ_GRP(double, 2);
+        else SPZ_GRP(double, 3);
+    } else {
+        if (nq == 1) SPZ_GRP(float, 1);
+        else if (nq == 2) SPZ_GRP(float, 2);
+        else SPZ_GRP(float, 3);
+    }
+#undef SPZ_GRP
+    SPZ_HIP(hipGetLastError());
+    return SPZ_OK;
 }
 
 struct DevBuf {
@@ -251,6 +337,13 @@ int adjoint_grad(spz_state_s *s, const spz_op *ops, uint64_t n_ops, double hx, i
             o.kind < SPZ_H || o.kind > SPZ_SWAP)
             return fail(SPZ_ERR_GATE, "adjoint: op " + std::to_string(i) +
                                           " is not a unitary single-target gate or SWAP");
+        const uint64_t all = s->n >= 64 ? ~0ull : (1ull << s->n) - 1;
+        const uint64_t tb = o.target >= 0 && o.target < s->n ? 1ull << o.target : 0;
+        const uint64_t t1b = o.t1 >= 0 && o.t1 < s->n ? 1ull << o.t1 : 0;
+        if (!tb || (o.kind == SPZ_SWAP && (!t1b || t1b == tb)) || ((o.ones | o.zeros) & ~all) ||
+            (o.ones & o.zeros) || ((o.ones | o.zeros) & (tb | (o.kind == SPZ_SWAP ? t1b : 0))))
+            return fail(SPZ_ERR_QUBIT, "adjoint: op " + std::to_string(i) +
+                                           " has a qubit out of range or used twice");
     }
     const int n = s->n;
     // ---- forward pass with checkpoints ---------------------------------------
@@ -299,54 +392,96 @@ int adjoint_grad(spz_state_s *s, const spz_op *ops, uint64_t n_ops, double hx, i
     DevBuf dgrad, dpart, dmax;
     SPZ_HIP(hipMalloc(&dgrad.p, std::max<uint64_t>(n_ops, 1) * sizeof(double)));
     SPZ_HIP(hipMemsetAsync(dgrad.p, 0, std::max<uint64_t>(n_ops, 1) * sizeof(double), rt().stream));
-    const uint64_t max_blocks = (s->N / 2 + (uint64_t)kT * kPerThread - 1) / ((uint64_t)kT * kPerThread) + 1;
-    SPZ_HIP(hipMalloc(&dpart.p, max_blocks * 2 * sizeof(double)));
+    const uint64_t per_block = (uint64_t)kT * kGPerThread;
+    const uint64_t max_blocks = (s->N / 2 + per_block - 1) / per_block + 1;
+    SPZ_HIP(hipMalloc(&dpart.p, max_blocks * 2 * kGMaxOps * sizeof(double)));
     SPZ_HIP(hipMalloc(&dmax.p, sizeof(unsigned long long)));
     SPZ_HIP(hipMemsetAsync(dmax.p, 0, sizeof(unsigned long long), rt().stream));
     double *gdev = (double *)dgrad.p, *part = (double *)dpart.p;
     const double amp = s->prec == SPZ_F64 ? 16.0 : 8.0;
+    // run_lo[k]: first index of the run of parameter-free ops that ends at k
+    std::vector<uint64_t> run_lo(n_ops);
+    for (uint64_t k = 0; k < n_ops; k++)
+        run_lo[k] = has_param(ops[k].kind) ? k + 1 : (k > 0 && !has_param(ops[k - 1].kind) ? run_lo[k - 1] : k);
+    const bool tiled = n >= 13;  // the fused executor tiles 12 qubits
+    constexpr uint64_t kMinRun = 4;
     int next_ck = (int)ckpt_at.size() - 1;
-    for (uint64_t ii = n_ops; ii-- > 0;) {
-        const spz_op &o = ops[ii];
-        if (o.kind == SPZ_SWAP) {  // self-inverse, no parameter
+    uint64_t ii = n_ops;  // ops [0, ii) remain
+    while (ii > 0) {
+        const uint64_t hi = ii - 1;
+        // a block of ops must not step over a checkpoint restore point
+        const uint64_t lo_lim = next_ck >= 0 ? ckpt_at[next_ck] + 1 : 0;
+        const spz_op &o = ops[hi];
+        uint64_t lo = hi;
+        if (tiled && !has_param(o.kind) && hi + 1 - std::max(run_lo[hi], lo_lim) >= kMinRun) {
+            lo = std::max(run_lo[hi], lo_lim);
+            std::vector<spz_op> inv;
+            inv.reserve(hi + 1 - lo);
+            for (uint64_t k = hi + 1; k-- > lo;) inv.push_back(inverse_op(ops[k]));
+            SPZ_TRY(execute_ops(work.s, inv.data(), inv.size(), nullptr, &eo, nullptr));
+            SPZ_TRY(execute_ops(lam.s, inv.data(), inv.size(), nullptr, &eo, nullptr));
+        } else if (o.kind == SPZ_SWAP) {  // self-inverse, no parameter
             SPZ_TRY(apply_gate(work.s, SPZ_SWAP, 0, 0, 0, o.target, o.t1, o.ones, o.zeros));
             SPZ_TRY(apply_gate(lam.s, SPZ_SWAP, 0, 0, 0, o.target, o.t1, o.ones, o.zeros));
         } else {
-            const int t = o.target;
-            if (t < 0 || t >= n) return fail(SPZ_ERR_QUBIT, "adjoint: target out of range");
-            // inverse gate (gates.rs:78-92)
-            double th = o.theta, ph = o.phi, la = o.lambda;
-            int kind = o.kind;
-            if (kind == SPZ_P || kind == SPZ_RX || kind == SPZ_RY || kind == SPZ_RZ) th = -th;
-            if (kind == SPZ_U) { th = -o.theta; ph = -o.lambda; la = -o.phi; }
-            Coef k;
-            SPZ_TRY(make_coef(kind, th, ph, la, &k));
-            int gk = GK_NONE;
-            if (kind == SPZ_RX) gk = GK_X;
-            else if (kind == SPZ_RY) gk = GK_Y;
-            else if (kind == SPZ_RZ) gk = GK_Z;
-            else if (kind == SPZ_P) gk = GK_P;
-            PosList pl;
-            const uint64_t fixed = o.ones | o.zeros | (1ull << t);
-            const int np = build_poslist(fixed, &pl);
-            if (np > n) return fail(SPZ_ERR_QUBIT, "adjoint: too many fixed qubits");
-            const uint64_t npairs = s->N >> np;
-            const unsigned g = blocks_for(npairs, (uint64_t)kT * kPerThread);
-            {
-                LaunchTimer tm(SPZ_K_ADJOINT, 2.0 * 2.0 * 2.0 * (double)npairs * amp);
-                if (s->prec == SPZ_F64)
-                    SPZ_TRY(step<double>(work.s, lam.s, npairs, t, o.ones, pl, k, gk, part, g));
-                else
-                    SPZ_TRY(step<float>(work.s, lam.s, npairs, t, o.ones, pl, k, gk, part, g));
+            // greedy group: ops hi, hi-1, ... while the targets fit in kGMaxQ qubits
+            GroupDesc gd;
+            memset(&gd, 0, sizeof gd);
+            GradIdx gi;
+            int nq = 0, ngrad = 0;
+            uint64_t qmask = 0, all_ones = ~0ull, all_zeros = ~0ull;
+            for (uint64_t k = hi + 1; k-- > lo_lim;) {
+                const spz_op &g = ops[k];
+                if (g.kind == SPZ_SWAP || gd.nop == kGMaxOps) break;
+                if (k != hi && tiled && !has_param(g.kind) &&
+                    k + 1 - std::max(run_lo[k], lo_lim) >= kMinRun)
+                    break;  // leave a long parameter-free run to the tile executor
+                const uint64_t tb = 1ull << g.target;
+                if (!(qmask & tb)) {
+                    if (nq == kGMaxQ) break;
+                    gd.q[nq++] = (int8_t)g.target;
+                    qmask |= tb;
+                }
+                GroupOp &go = gd.op[gd.nop];
+                double th = g.theta, ph = g.phi, la = g.lambda;
+                if (has_param(g.kind)) th = -th;
+                if (g.kind == SPZ_U) { th = -g.theta; ph = -g.lambda; la = -g.phi; }  // gates.rs:78-92
+                Coef kc;
+                SPZ_TRY(make_coef(g.kind, th, ph, la, &kc));
+                go.ak = kc.ak;
+                memcpy(go.c, kc.c, sizeof go.c);
+                go.gk = grad_kind(g.kind);
+                for (int b = 0; b < nq; b++)
+                    if (gd.q[b] == g.target) go.tq = b;
+                go.cm = g.ones | g.zeros;
+                go.ones = g.ones;
+                all_ones &= g.ones;
+                all_zeros &= g.zeros;
+                gi.i[gd.nop] = go.gk != GK_NONE ? (int64_t)k : -1;
+                ngrad += go.gk != GK_NONE;
+                gd.nop++;
+                lo = k;
             }
-            if (gk != GK_NONE) {
-                hipLaunchKernelGGL(k_adj_finish, dim3(1), dim3(kT), 0, rt().stream, part,
-                                   (uint64_t)g, gdev + ii);
+            // common controls outside Q are fixed positions of every element
+            all_ones &= ~qmask;
+            all_zeros &= ~qmask;
+            gd.base_ones = all_ones;
+            PosList pl;
+            const int np = build_poslist(qmask | all_ones | all_zeros, &pl);
+            const uint64_t nel = s->N >> np;
+            const unsigned g = blocks_for(nel, per_block);
+            {
+                LaunchTimer tm(SPZ_K_ADJOINT, 2.0 * 2.0 * (double)(nel << nq) * amp);
+                SPZ_TRY(launch_group(work.s, lam.s, nq, nel, pl, gd, part, g));
+            }
+            if (ngrad) {  // one block per op; ops without a gradient write nothing
+                hipLaunchKernelGGL(k_adj_finish, dim3(gd.nop), dim3(kT), 0, rt().stream, part,
+                                   (uint64_t)g, gi, gdev);
                 SPZ_HIP(hipGetLastError());
             }
         }
-        // work now holds psi_{ii-1}: restore from a checkpoint taken after op ii-1
-        if (next_ck >= 0 && ii >= 1 && ckpt_at[next_ck] == ii - 1) {
+        // work now holds psi_{lo-1}: restore from a checkpoint taken after op lo-1
+        if (next_ck >= 0 && lo >= 1 && ckpt_at[next_ck] == lo - 1) {
             spz_state_s *ck = ckpts[next_ck]->s;
             if (s->prec == SPZ_F64)
                 hipLaunchKernelGGL((k_maxdiff<double>), dim3(blocks_for(s->N, kT)), dim3(kT), 0,
@@ -362,6 +497,7 @@ int adjoint_grad(spz_state_s *s, const spz_op *ops, uint64_t n_ops, double hx, i
             SPZ_TRY(spz_state_copy(work.s, ck));
             next_ck--;
         }
+        ii = lo;
     }
     SPZ_HIP(hipMemcpyAsync(grad, gdev, n_ops * sizeof(double), hipMemcpyDeviceToHost, rt().stream));
     unsigned long long mx = 0;

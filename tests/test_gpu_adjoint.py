@@ -190,3 +190,51 @@ def test_batched_adjoint_matches_single(spz):
             e1, g1, _ = adjoint_gradient(spz.State(n), ops, 0.8)
             assert abs(e - e1) < 1e-12
             assert np.max(np.abs(g - g1)) < 1e-12
+
+
+def test_adjoint_grouped_and_tiled_reverse_sweep(spz, orc):
+    """14 qubits: long parameter-free runs (undone by the tile executor), rotation
+    groups on up to 3 qubits with controls inside and outside the group, SWAPs
+    and a checkpoint every 17 ops; energy against the oracle and every gradient
+    against central finite differences of the oracle energy."""
+    print("case adjoint grouped/tiled 14q", flush=True)
+    from spinoza_amd import _native as N
+    from spinoza_amd.autograd import adjoint_gradient
+    from spinoza_amd.workloads import op
+    orc.set_threads(16)
+    n = 14
+    rng = np.random.default_rng(11)
+    ops = []
+    for layer in range(3):
+        for q in range(n):
+            ops.append(op(N.RY, q, theta=rng.uniform(0, 6.28)))
+            ops.append(op(N.RZ, q, theta=rng.uniform(0, 6.28)))
+        for q in range(0, n - 1, 2):  # parameter-free run: CX, H, CCX, U, SWAP, Y, Z
+            ops.append(op(N.X, q + 1, ones=1 << q))
+            ops.append(op(N.H, (q + 5) % n))
+        ops.append(op(N.X, 13, ones=(1 << 2) | (1 << 9)))
+        ops.append(op(N.U, 6, theta=0.3, phi=0.8, lam=-0.5))
+        ops.append(op(N.SWAP, 3, t1=12))
+        ops.append(op(N.Y, 7))
+        ops.append(op(N.Z, 0, ones=1 << 11))
+        ops.append(op(N.RX, 2, theta=0.4, ones=1 << 3))            # control inside a group
+        ops.append(op(N.RY, 3, theta=1.3, ones=1 << 2))
+        ops.append(op(N.RZ, 4, theta=-0.7, ones=(1 << 10) | (1 << 12)))  # controls outside
+        ops.append(op(N.P, 2, theta=0.9, zeros=1 << 13))
+        ops.append(op(N.RX, 9, theta=2.1, ones=1 << 0))
+    s = spz.State(n)
+    e, g, drift = adjoint_gradient(s, ops, 0.6, ckpt_every=17)
+    assert drift < 1e-12
+    assert abs(e - oracle_energy(orc, n, ops, 0.6)) < 1e-12
+    h = 1e-5
+    params = [i for i, o in enumerate(ops) if o["kind"] in (N.RX, N.RY, N.RZ, N.P)]
+    for i in range(len(ops)):
+        if i not in params:
+            assert g[i] == 0.0
+    for i in params[::3] + params[-6:]:
+        plus = [dict(x) for x in ops]
+        minus = [dict(x) for x in ops]
+        plus[i]["theta"] += h
+        minus[i]["theta"] -= h
+        fd = (oracle_energy(orc, n, plus, 0.6) - oracle_energy(orc, n, minus, 0.6)) / (2 * h)
+        assert abs(g[i] - fd) < 1e-8, (i, ops[i], g[i], fd)
