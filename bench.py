@@ -9,11 +9,21 @@ Prints ONE JSON line:
   roofline   the dominant kernel class: algorithmic HBM bytes / device time,
              measured with HIP events on the library stream over the timed
              region, against the 8 TB/s HBM peak
-  cpu_baseline  the oracle (C restatement of the reference's rayon path,
-             OpenMP) timed on this host on a bounded sample of the workload
+  adjoint    24-qubit, 480-parameter adjoint gradient: grads/s and the ratio
+             against parameter shift on the GPU
+  fusion     60-layer random circuits at 32 q FP64 and 34 q FP32 through the
+             tile executor and the dense 5-qubit MFMA blocks, with fidelity,
+             norm and <Z0 Z_n-1> errors against an unfused FP64 run at 28 q
+  batch      1024 independent small-circuit adjoint gradients (n in 10..20,
+             depth 2..12): p50/p99 latency and grads/s against a loop of
+             single-circuit calls
+  qft        20-qubit QFT of a Gaussian state plus 4096 samples
   per_stride plain-kernel GB/s per target qubit for the gate classes
-  replica    max |amplitude error| of the GPU run against the oracle on the
-             26-qubit replica of the circuit (first few layers)
+  cpu_baseline  the oracle (C restatement of the reference's rayon path,
+             OpenMP) timed on this host on a bounded sample of the workload;
+             the same leg uses the oracle as the checker of the GPU results
+             (26-qubit replica, QFT amplitudes and samples, adjoint gradient
+             against parameter shift)
 """
 import argparse
 import json
@@ -26,6 +36,8 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK = 8.0e12
+MFMA_PEAK_F64 = 78.6e12   # dense FP64 matrix peak
+MFMA_PEAK_F32 = 157.3e12  # dense FP32 matrix peak
 
 
 def parse():
@@ -50,6 +62,16 @@ def parse():
                         "writes every byte of the state once (counter calibration)")
     p.add_argument("--ps-subset", type=int, default=24,
                    help="parameters timed with GPU parameter shift (extrapolated to all)")
+    p.add_argument("--cpu-ps-subset", type=int, default=4,
+                   help="24q adjoint gradients checked against oracle parameter shift")
+    p.add_argument("--no-fusion", action="store_true", help="skip the 32q/34q fusion leg")
+    p.add_argument("--fusion-layers", type=int, default=60)
+    p.add_argument("--fusion-qubits", type=str, default="32,34",
+                   help="FP64 and FP32 qubit counts of the fusion leg")
+    p.add_argument("--precision-qubits", type=int, default=28)
+    p.add_argument("--no-batch", action="store_true", help="skip the batched adjoint leg")
+    p.add_argument("--batch-count", type=int, default=1024)
+    p.add_argument("--no-qft", action="store_true", help="skip the 20q QFT leg")
     return p.parse_args()
 
 
@@ -132,7 +154,157 @@ def adjoint_leg(spz, N, n, layers, ps_subset, seed=4):
             "param_shift_s_full_extrapolated": round(t_ps_per_param * len(where), 3),
             "speedup_vs_param_shift": round(t_ps_per_param * len(where) / t_adj, 1),
             "param_shift_subset": len(sub), "max_abs_grad_err_vs_param_shift": err,
-            "checkpoint_drift": drift}
+            "checkpoint_drift": drift}, (n, ops, where, g)
+
+
+def timed(fn, sync):
+    sync()
+    t0 = time.perf_counter()
+    out = fn()
+    sync()
+    return time.perf_counter() - t0, out
+
+
+def fusion_leg(spz, N, layers, n64, n32, n_prec, seed=3):
+    """Random circuits (the 30q generator) with `layers` layers.
+
+    Precision at n_prec qubits: FP64 and FP32 storage through the tile
+    executor and the dense 5-qubit blocks against the unfused FP64 plain run:
+    fidelity |<psi|psi_ref>|^2, |norm - norm_ref| and |<Z0 Z_n-1> - ref|.
+    Throughput at n64 qubits FP64 and n32 qubits FP32 for both executors, with
+    the dense-block kernel's HBM and MFMA fractions."""
+    import ctypes
+    from spinoza_amd.executor import encode_ops, run_ops
+    from spinoza_amd.workloads import random_circuit
+    sync = lambda: N.check(N.lib().spz_sync())
+    modes = {"tile": N.EXEC_AUTO, "dense5": N.EXEC_DENSE5}
+    out = {"layers": layers, "precision": {"qubits": n_prec}}
+    ops = random_circuit(n_prec, layers, seed)
+    ref = spz.State(n_prec, "f64")
+    run_ops(ref, ops, mode=N.EXEC_PLAIN)
+    ref_norm = ref.norm()
+    ref_zz = spz.expect_zz(ref, 0, n_prec - 1)
+    for prec in ("f64", "f32"):
+        for name, mode in modes.items():
+            s = spz.State(n_prec, prec)
+            run_ops(s, ops, mode=mode)
+            nrm = s.norm()
+            # state fidelity of the normalised states; the norm error is reported apart
+            fid = abs(spz.inner(s, ref)) ** 2 / (nrm * ref_norm)
+            out["precision"][f"{name}_{prec}"] = {
+                "fidelity": fid, "infidelity": 1.0 - fid,
+                "norm_err": abs(nrm - ref_norm),
+                "zz_err": abs(spz.expect_zz(s, 0, n_prec - 1) - ref_zz)}
+            del s
+    del ref
+    for n, prec in ((n64, "f64"), (n32, "f32")):
+        ops = random_circuit(n, layers, seed)
+        enc = encode_ops(ops)
+        s = spz.State(n, prec)
+        row = {"gates": len(ops), "state_bytes": (16 if prec == "f64" else 8) << n}
+        for name, mode in modes.items():
+            s.set_basis(0)
+            N.check(N.lib().spz_profile_enable(1))
+            N.check(N.lib().spz_profile_reset())
+            dt, _ = timed(lambda: run_ops(s, ops, mode=mode, encoded=enc), sync)
+            cnt, ms, by = ctypes.c_uint64(), ctypes.c_double(), ctypes.c_double()
+            cls = N.K_TILE if name == "tile" else N.K_DENSE
+            N.check(N.lib().spz_profile_get(cls, ctypes.byref(cnt), ctypes.byref(ms),
+                                            ctypes.byref(by)))
+            N.check(N.lib().spz_profile_enable(0))
+            r = {"s": round(dt, 3), "gates_per_s": round(len(ops) / dt, 1),
+                 "launches": cnt.value, "kernel_ms": round(ms.value, 1),
+                 "hbm_frac": round(by.value / (ms.value * 1e-3) / HBM_PEAK, 4) if ms.value else None,
+                 "norm_err": abs(s.norm() - 1.0),
+                 "zz": spz.expect_zz(s, 0, n - 1)}
+            if name == "dense5" and ms.value:
+                # 32x32 complex block: 8 * 32 real flops per amplitude
+                flops = cnt.value * 256.0 * (1 << n)
+                peak = MFMA_PEAK_F64 if prec == "f64" else MFMA_PEAK_F32
+                r["mfma_tflops"] = round(flops / (ms.value * 1e-3) / 1e12, 2)
+                r["mfma_frac"] = round(flops / (ms.value * 1e-3) / peak, 4)
+            row[name] = r
+        out[f"{n}q_{prec}"] = row
+        del s
+    return out
+
+
+def batch_leg(spz, N, count, seed=5):
+    """`count` hardware-efficient-ansatz circuits, n ~ U{10..20}, depth ~
+    U{2..12} (SplitMix64 seed 5), Ising energy (hx = 1): one batched call
+    after autotuning the per-workgroup switch point, against looping the
+    single-circuit adjoint."""
+    import numpy as np
+    from spinoza_amd.autograd import adjoint_gradient, batch_adjoint, batch_autotune
+    from spinoza_amd.rng import SplitMix64
+    from spinoza_amd.workloads import hea
+    sync = lambda: N.check(N.lib().spz_sync())
+    rng = SplitMix64(seed)
+    circuits, nparams = [], 0
+    for i in range(count):
+        n = 10 + rng.next_u64() % 11
+        depth = 2 + rng.next_u64() % 11
+        ops, _, where = hea(n, depth, seed=seed * 100003 + i)
+        circuits.append((n, ops))
+        nparams += len(where)
+    wg = batch_autotune(circuits)
+    batch_adjoint(circuits, 1.0)  # warm (graph capture, buffers)
+    dt, (energies, grads, st) = timed(lambda: batch_adjoint(circuits, 1.0), sync)
+    states = {}
+    for n, _ in circuits:
+        if n not in states:
+            states[n] = spz.State(n)
+    adjoint_gradient(states[circuits[0][0]], circuits[0][1], 1.0)  # warm
+    err = 0.0
+
+    def loop():
+        res = []
+        for n, ops in circuits:
+            states[n].set_basis(0)
+            res.append(adjoint_gradient(states[n], ops, 1.0))
+        return res
+
+    dt_loop, res = timed(loop, sync)
+    for (e1, g1, _), e, g in zip(res, energies, grads):
+        err = max(err, abs(e - e1), float(np.max(np.abs(g - g1))))
+    del states
+    return {"circuits": count, "params": nparams, "wg_max_qubits": wg,
+            "n_small": st["n_small"], "n_large": st["n_large"],
+            "total_ms": round(dt * 1e3, 2), "p50_ms": round(st["p50_ms"], 4),
+            "p99_ms": round(st["p99_ms"], 4), "grads_per_s": round(nparams / dt, 1),
+            "loop_single_ms": round(dt_loop * 1e3, 2),
+            "loop_grads_per_s": round(nparams / dt_loop, 1),
+            "speedup_vs_loop": round(dt_loop / dt, 2), "max_abs_diff_vs_single": err}
+
+
+def qft_leg(spz, N, n=20, seed=1):
+    """QFT (H + controlled phases, MSB first, then the swap layer) of a
+    normalised Gaussian state (SplitMix64 seed 1, Box-Muller) and 4096
+    inverse-CDF samples from SplitMix64 seed 2 uniforms.  Returns the GPU
+    timings and the device state for the checker."""
+    from spinoza_amd.executor import encode_ops, run_ops
+    from spinoza_amd.rng import gaussian_state, uniforms
+    from spinoza_amd.workloads import qft
+    sync = lambda: N.check(N.lib().spz_sync())
+    ops = qft(n)
+    enc = encode_ops(ops)
+    re, im = gaussian_state(seed, n)
+    src = spz.State.from_arrays(re, im)
+    s = spz.State(n)
+    u = uniforms(seed + 1, 4096)
+    s.copy_from(src)
+    run_ops(s, ops, encoded=enc)  # warm
+    reps = 20
+
+    def run():
+        for _ in range(reps):
+            s.copy_from(src)
+            run_ops(s, ops, encoded=enc)
+
+    dt, _ = timed(run, sync)
+    dts, smp = timed(lambda: spz.sample(s, u), sync)
+    return {"qubits": n, "gates": len(ops), "ms_per_qft": round(dt * 1e3 / reps, 4),
+            "ms_per_4096_samples": round(dts * 1e3, 4)}, (s, smp, u, re, im, ops)
 
 
 def cpu_baseline(ops, n, threads, budget_s=20.0):
@@ -240,36 +412,38 @@ def main():
     }
     del s
 
+    adj_ctx = None
     if not a.no_adjoint:
         log(f"adjoint leg: {a.adjoint_qubits}q, {a.adjoint_layers} layers")
-        result["adjoint"] = adjoint_leg(spz, N, a.adjoint_qubits, a.adjoint_layers, a.ps_subset)
+        result["adjoint"], adj_ctx = adjoint_leg(spz, N, a.adjoint_qubits, a.adjoint_layers,
+                                                 a.ps_subset)
+
+    if not a.no_fusion:
+        n64, n32 = (int(x) for x in a.fusion_qubits.split(","))
+        log(f"fusion leg: {n64}q FP64, {n32}q FP32, {a.fusion_layers} layers; "
+            f"precision at {a.precision_qubits}q")
+        result["fusion"] = fusion_leg(spz, N, a.fusion_layers, n64, n32, a.precision_qubits)
+
+    if not a.no_batch:
+        log(f"batched adjoint leg: {a.batch_count} circuits")
+        result["batch"] = batch_leg(spz, N, a.batch_count)
+
+    qft_ctx = None
+    if not a.no_qft:
+        log("qft leg: 20q")
+        result["qft"], qft_ctx = qft_leg(spz, N)
 
     if not a.no_strides:
         log("per-stride sweep")
         result["per_stride_gb_s"] = per_stride(spz, N, n)
 
-    if not a.no_replica:
-        from oracle import pyoracle
+    if not a.no_cpu:
+        # cpu_baseline leg: the oracle as the timed CPU baseline and as the
+        # checker of the GPU results above
         import numpy as np
-        rn = 26
-        rops = random_circuit(rn, a.replica_layers, a.seed)
+        from oracle import pyoracle
         threads = a.cpu_threads or min(16, os.cpu_count() or 1)
         pyoracle.set_threads(threads)
-        log(f"26q replica: {len(rops)} gates on the oracle ({threads} threads)")
-        ref = pyoracle.OracleState(rn)
-        pyoracle.execute(ref, rops)
-        g = spz.State(rn)
-        run_ops(g, rops, mode=mode)
-        re, im = g.download()
-        err = float(max(np.max(np.abs(re - ref.re)), np.max(np.abs(im - ref.im))))
-        result["replica"] = {"qubits": rn, "layers": a.replica_layers, "gates": len(rops),
-                             "max_abs_err": err,
-                             "bit_identical": bool(np.array_equal(re, ref.re) and
-                                                   np.array_equal(im, ref.im))}
-        del g, ref
-
-    if not a.no_cpu:
-        threads = a.cpu_threads or min(16, os.cpu_count() or 1)
         log(f"cpu baseline: oracle, {threads} threads, first gates of the {n}q workload")
         done, dt = cpu_baseline(ops, n, threads)
         result["cpu_baseline"] = {
@@ -278,6 +452,54 @@ def main():
             "sample": f"first {done} gates of the same {n}q circuit ({dt:.1f} s), "
                       f"OpenMP restatement of the reference's rayon pair loops"}
         result["speedup_vs_cpu"] = round(value / (done / dt), 1)
+        checks = {}
+        if not a.no_replica:
+            rn = 26
+            rops = random_circuit(rn, a.replica_layers, a.seed)
+            log(f"check: 26q replica, {len(rops)} gates on the oracle")
+            ref = pyoracle.OracleState(rn)
+            pyoracle.execute(ref, rops)
+            g = spz.State(rn)
+            run_ops(g, rops, mode=mode)
+            re, im = g.download()
+            checks["replica_26q"] = {
+                "layers": a.replica_layers, "gates": len(rops),
+                "max_abs_err": float(max(np.max(np.abs(re - ref.re)), np.max(np.abs(im - ref.im)))),
+                "bit_identical": bool(np.array_equal(re, ref.re) and np.array_equal(im, ref.im))}
+            del g, ref
+        if qft_ctx is not None:
+            s20, smp, u, re0, im0, qops = qft_ctx
+            log("check: 20q QFT and samples on the oracle")
+            ref = pyoracle.OracleState(20)
+            ref.re[:] = re0
+            ref.im[:] = im0
+            t0 = time.perf_counter()
+            pyoracle.execute(ref, qops)
+            t_cpu = time.perf_counter() - t0
+            re, im = s20.download()
+            checks["qft_20q"] = {
+                "max_abs_err": float(max(np.max(np.abs(re - ref.re)), np.max(np.abs(im - ref.im)))),
+                "samples_bit_identical": bool(np.array_equal(smp, ref.sample(u))),
+                "oracle_ms": round(t_cpu * 1e3, 1)}
+            del ref
+        if adj_ctx is not None and a.cpu_ps_subset > 0:
+            an, aops, where, g = adj_ctx
+            sub = where[::max(1, len(where) // a.cpu_ps_subset)][:a.cpu_ps_subset]
+            log(f"check: {an}q adjoint against oracle parameter shift ({len(sub)} parameters)")
+            err = 0.0
+            for i in sub:
+                e = []
+                for sh in (math.pi / 2, -math.pi / 2):
+                    sops = [dict(o) for o in aops]
+                    sops[i]["theta"] += sh
+                    st = pyoracle.OracleState(an)
+                    pyoracle.execute(st, sops)
+                    e.append(st.ising_energy(1.0))
+                    del st
+                err = max(err, abs(g[i] - 0.5 * (e[0] - e[1])))
+            checks["adjoint_vs_cpu_param_shift"] = {"qubits": an, "params": len(sub),
+                                                   "max_abs_err": err}
+        result["checks"] = checks
 
     print(json.dumps(result), flush=True)
 

@@ -196,7 +196,8 @@ int spz_xyz_expectation_value(spz_state s, char observable, const int32_t *targe
 int spz_expect_zz(spz_state s, int q0, int q1, double *out);
 /* <H> for H = sum_{i<n-1} Z_i Z_{i+1} + hx * sum_i X_i (open chain). */
 int spz_ising_energy(spz_state s, double hx, double *out);
-/* <a|b> (both states of equal size and precision). */
+/* <a|b> for two states of equal size; their storage precisions may differ
+   (an FP32 state against an FP64 one: the terms are formed and summed in FP64). */
 int spz_inner(spz_state a, spz_state b, double *re, double *im);
 /* Replaces reservoir_sampling (core.rs:125-129): k indices drawn from
  * |psi|^2 by the inverse-CDF rule, one per uniform in [0,1). */

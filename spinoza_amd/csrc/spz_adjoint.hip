@@ -134,30 +134,34 @@ k_adj_group(T *__restrict__ pr, T *__restrict__ pi, T *__restrict__ qr, T *__res
             if (x & (1 << b)) o |= 1ull << gd.q[b];
         off[x] = o;
     }
-    const uint64_t block0 = (uint64_t)blockIdx.x * kT * kGPerThread;
-    for (int w = 0; w < kGPerThread; w++) {
-        const uint64_t u = block0 + (uint64_t)w * kT + tid;
-        if (u >= nel) break;
-        const uint64_t base = insert_zeros(u, pl.pos, pl.np) | gd.base_ones;
-        double ar[1 << M], ai[1 << M], lr[1 << M], li[1 << M];
+    // blocks stride over chunks of kT * kGPerThread elements
+    const uint64_t chunk = (uint64_t)kT * kGPerThread;
+    for (uint64_t block0 = (uint64_t)blockIdx.x * chunk; block0 < nel;
+         block0 += (uint64_t)gridDim.x * chunk) {
+        for (int w = 0; w < kGPerThread; w++) {
+            const uint64_t u = block0 + (uint64_t)w * kT + tid;
+            if (u >= nel) break;
+            const uint64_t base = insert_zeros(u, pl.pos, pl.np) | gd.base_ones;
+            double ar[1 << M], ai[1 << M], lr[1 << M], li[1 << M];
 #pragma unroll
-        for (int x = 0; x < (1 << M); x++) {
-            const uint64_t i = base | off[x];
-            ar[x] = pr[i]; ai[x] = pi[i]; lr[x] = qr[i]; li[x] = qi[i];
-        }
-        for (int j = 0; j < nop; j++) {
-            const GroupOp &op = gd.op[j];
-            double s = S[j][tid], e = C[j][tid];
-            if (op.tq == 0) group_apply<M, 0>(op, base, off, ar, ai, lr, li, s, e);
-            if constexpr (M > 1) if (op.tq == 1) group_apply<M, 1>(op, base, off, ar, ai, lr, li, s, e);
-            if constexpr (M > 2) if (op.tq == 2) group_apply<M, 2>(op, base, off, ar, ai, lr, li, s, e);
-            S[j][tid] = s;
-            C[j][tid] = e;
-        }
+            for (int x = 0; x < (1 << M); x++) {
+                const uint64_t i = base | off[x];
+                ar[x] = pr[i]; ai[x] = pi[i]; lr[x] = qr[i]; li[x] = qi[i];
+            }
+            for (int j = 0; j < nop; j++) {
+                const GroupOp &op = gd.op[j];
+                double s = S[j][tid], e = C[j][tid];
+                if (op.tq == 0) group_apply<M, 0>(op, base, off, ar, ai, lr, li, s, e);
+                if constexpr (M > 1) if (op.tq == 1) group_apply<M, 1>(op, base, off, ar, ai, lr, li, s, e);
+                if constexpr (M > 2) if (op.tq == 2) group_apply<M, 2>(op, base, off, ar, ai, lr, li, s, e);
+                S[j][tid] = s;
+                C[j][tid] = e;
+            }
 #pragma unroll
-        for (int x = 0; x < (1 << M); x++) {
-            const uint64_t i = base | off[x];
-            pr[i] = (T)ar[x]; pi[i] = (T)ai[x]; qr[i] = (T)lr[x]; qi[i] = (T)li[x];
+            for (int x = 0; x < (1 << M); x++) {
+                const uint64_t i = base | off[x];
+                pr[i] = (T)ar[x]; pi[i] = (T)ai[x]; qr[i] = (T)lr[x]; qi[i] = (T)li[x];
+            }
         }
     }
     __syncthreads();
@@ -219,18 +223,19 @@ __global__ void __launch_bounds__(kT)
 k_ising_apply(const T *__restrict__ re, const T *__restrict__ im, T *__restrict__ lr,
               T *__restrict__ li, uint64_t N, int n, double hx)
 {
-    const uint64_t j = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (j >= N) return;
-    int czz = 0;
-    for (int i = 0; i + 1 < n; i++) czz += (((j >> i) ^ (j >> (i + 1))) & 1) ? -1 : 1;
-    double xr = 0.0, xi = 0.0;
-    for (int i = 0; i < n; i++) {
-        const uint64_t p = j ^ (1ull << i);
-        xr = xr + (double)re[p];
-        xi = xi + (double)im[p];
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t j = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; j < N; j += stride) {
+        int czz = 0;
+        for (int i = 0; i + 1 < n; i++) czz += (((j >> i) ^ (j >> (i + 1))) & 1) ? -1 : 1;
+        double xr = 0.0, xi = 0.0;
+        for (int i = 0; i < n; i++) {
+            const uint64_t p = j ^ (1ull << i);
+            xr = xr + (double)re[p];
+            xi = xi + (double)im[p];
+        }
+        lr[j] = (T)((double)czz * (double)re[j] + hx * xr);
+        li[j] = (T)((double)czz * (double)im[j] + hx * xi);
     }
-    lr[j] = (T)((double)czz * (double)re[j] + hx * xr);
-    li[j] = (T)((double)czz * (double)im[j] + hx * xi);
 }
 
 // max |a - b| over the state, into out[0] via an integer atomic max on the
@@ -240,16 +245,21 @@ __global__ void __launch_bounds__(kT)
 k_maxdiff(const T *__restrict__ ar, const T *__restrict__ ai, const T *__restrict__ br,
           const T *__restrict__ bi, uint64_t N, unsigned long long *__restrict__ out)
 {
-    const uint64_t j = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
     double m = 0.0;
-    if (j < N) m = fmax(fabs((double)ar[j] - (double)br[j]), fabs((double)ai[j] - (double)bi[j]));
+    for (uint64_t j = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; j < N; j += stride)
+        m = fmax(m, fmax(fabs((double)ar[j] - (double)br[j]), fabs((double)ai[j] - (double)bi[j])));
     for (int off = 32; off >= 1; off >>= 1) m = fmax(m, __shfl_xor(m, off));
     if ((threadIdx.x & 63) == 0) atomicMax(out, (unsigned long long)__double_as_longlong(m));
 }
 
+// Blocks of kT threads covering `units` (per_block units each), capped at the
+// dispatch limit of 2^32 - 1 work-items (element-wise kernels grid-stride).
 inline unsigned blocks_for(uint64_t units, uint64_t per_block)
 {
     uint64_t g = (units + per_block - 1) / per_block;
+    const uint64_t cap = 0xFFFFFFFFull / kT;
+    if (g > cap) g = cap;
     return (unsigned)(g == 0 ? 1 : g);
 }
 

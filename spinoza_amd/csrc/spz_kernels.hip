@@ -13,6 +13,8 @@
 // about 2048 workgroups keeps all 256 CUs busy for any state size.
 #include <stdlib.h>
 
+#include <type_traits>
+
 #include "spz_internal.h"
 
 namespace spz {
@@ -50,10 +52,12 @@ template <> struct Vec2<float> { using type = float2; };
 // Exact grids (one unit per thread) stream HBM markedly faster on MI355X than
 // capped grid-stride launches (measured 5.5-6.0 vs 4.8-5.0 TB/s for paired
 // streams); the grid-stride loop only engages beyond 2^30 workgroups.
+// One unit per thread up to the dispatch limit (a grid of at most 2^32 - 1
+// work-items); beyond it the grid-stride loops of the kernels take over.
 inline unsigned grid_for(uint64_t units)
 {
     uint64_t g = (units + kThreads - 1) / kThreads;
-    const uint64_t cap = 1ull << 30;
+    const uint64_t cap = 0xFFFFFFFFull / kThreads;
     if (g > cap) g = cap;
     if (g == 0) g = 1;
     return (unsigned)g;
@@ -248,9 +252,9 @@ __device__ __forceinline__ void two_sum(double a, double b, double &s, double &e
     s = ss;
 }
 
-template <typename T, int TERM>
+template <typename T, typename T2, int TERM>
 __device__ __forceinline__ double term(const T *__restrict__ re, const T *__restrict__ im,
-                                       const T *__restrict__ re2, const T *__restrict__ im2,
+                                       const T2 *__restrict__ re2, const T2 *__restrict__ im2,
                                        uint64_t j, int q0, int q1, int n, double hx)
 {
     if constexpr (TERM == TERM_NORM) {
@@ -317,10 +321,12 @@ __device__ __forceinline__ void tree256(double *S, double *C)
     }
 }
 
-template <typename T, int TERM>
+// T2: storage type of the second state (dot products of an FP32 and an FP64
+// state are summed in FP64 like the rest).
+template <typename T, typename T2, int TERM>
 __global__ void __launch_bounds__(kThreads)
-k_reduce1(const T *__restrict__ re, const T *__restrict__ im, const T *__restrict__ re2,
-          const T *__restrict__ im2, uint64_t M, int q0, int q1, int n, double hx,
+k_reduce1(const T *__restrict__ re, const T *__restrict__ im, const T2 *__restrict__ re2,
+          const T2 *__restrict__ im2, uint64_t M, int q0, int q1, int n, double hx,
           double *__restrict__ partials)
 {
     __shared__ double S[kThreads], C[kThreads];
@@ -331,7 +337,7 @@ k_reduce1(const T *__restrict__ re, const T *__restrict__ im, const T *__restric
     for (int e = 0; e < kRE; e++) {
         uint64_t j = base + (uint64_t)e * kThreads + tid;
         if (j < M) {
-            double x = term<T, TERM>(re, im, re2, im2, j, q0, q1, n, hx);
+            double x = term<T, T2, TERM>(re, im, re2, im2, j, q0, q1, n, hx);
             double ns, er;
             two_sum(s, x, ns, er);
             s = ns;
@@ -611,17 +617,24 @@ int launch_dense(spz_state_s *s, int m, const double *dur, const double *dui, in
 // ---------------------------------------------------------------------------
 // Reductions
 // ---------------------------------------------------------------------------
-template <typename T>
+template <typename T, typename T2 = T>
 static int reduce_launch(const spz_state_s *a, const spz_state_s *b, int term, int q0, int q1,
                          double hx, uint64_t M, uint64_t nb)
 {
     const T *re = (const T *)a->re, *im = (const T *)a->im;
-    const T *re2 = b ? (const T *)b->re : nullptr, *im2 = b ? (const T *)b->im : nullptr;
+    const T2 *re2 = b ? (const T2 *)b->re : nullptr, *im2 = b ? (const T2 *)b->im : nullptr;
     hipStream_t st = rt().stream;
     double *part = rt().d_partials;
     dim3 g((unsigned)nb), blk(kThreads);
     int n = a->n;
-#define SPZ_RL(TM) hipLaunchKernelGGL((k_reduce1<T, TM>), g, blk, 0, st, re, im, re2, im2, M, q0, q1, n, hx, part)
+#define SPZ_RL(TM) hipLaunchKernelGGL((k_reduce1<T, T2, TM>), g, blk, 0, st, re, im, re2, im2, M, q0, q1, n, hx, part)
+    if constexpr (!std::is_same<T, T2>::value) {  // mixed precision: dot products only
+        if (term == TERM_DOT_RE) SPZ_RL(TERM_DOT_RE);
+        else if (term == TERM_DOT_IM) SPZ_RL(TERM_DOT_IM);
+        else return fail(SPZ_ERR_ARG, "mixed-precision reduction other than a dot product");
+        SPZ_HIP(hipGetLastError());
+        return SPZ_OK;
+    }
     switch (term) {
     case TERM_NORM: SPZ_RL(TERM_NORM); break;
     case TERM_PROB0: SPZ_RL(TERM_PROB0); break;
@@ -647,9 +660,14 @@ static int reduce_level1(const spz_state_s *a, const spz_state_s *b, int term, i
     if (nb == 0) nb = 1;
     SPZ_TRY(ensure_partials(nb));
     double per_term = amp_bytes(a) * (term == TERM_X || term == TERM_Y ? 2.0 : 1.0);
-    if (term == TERM_DOT_RE || term == TERM_DOT_IM) per_term = 2.0 * amp_bytes(a);
+    if (term == TERM_DOT_RE || term == TERM_DOT_IM) per_term = amp_bytes(a) + amp_bytes(b);
     LaunchTimer tm(SPZ_K_REDUCE, (double)M * per_term);
-    int rc = a->prec == SPZ_F64 ? reduce_launch<double>(a, b, term, q0, q1, hx, M, nb)
+    int rc;
+    if (b && b->prec != a->prec)
+        rc = a->prec == SPZ_F64 ? reduce_launch<double, float>(a, b, term, q0, q1, hx, M, nb)
+                                : reduce_launch<float, double>(a, b, term, q0, q1, hx, M, nb);
+    else
+        rc = a->prec == SPZ_F64 ? reduce_launch<double>(a, b, term, q0, q1, hx, M, nb)
                                 : reduce_launch<float>(a, b, term, q0, q1, hx, M, nb);
     *nb_out = nb;
     return rc;

@@ -544,7 +544,7 @@ int spz_inner(spz_state a, spz_state b, double *re, double *im)
 {
     SPZ_TRY(check_state(a));
     SPZ_TRY(check_state(b));
-    if (a->n != b->n || a->prec != b->prec) return fail(SPZ_ERR_ARG, "inner product of different shapes");
+    if (a->n != b->n) return fail(SPZ_ERR_ARG, "inner product of states of different sizes");
     if (re) SPZ_TRY(reduce(a, b, TERM_DOT_RE, 0, 0, 0.0, re));
     if (im) SPZ_TRY(reduce(a, b, TERM_DOT_IM, 0, 0, 0.0, im));
     return SPZ_OK;

@@ -41,6 +41,25 @@ class SplitMix64:
         return (z >> np.uint64(11)).astype(np.float64) * (1.0 / (1 << 53))
 
 
+def uniforms(seed, count):
+    """`count` uniforms of the SplitMix64 stream that starts at `seed`."""
+    return SplitMix64(seed).uniforms(count)
+
+
+def gaussian_state(seed, n):
+    """(re, im) of a normalised 2^n-amplitude state with N(0, 1) parts
+    (Box-Muller on consecutive SplitMix64 pairs: u1 = 1 - u_{2i}, u2 = u_{2i+1},
+    re = r cos(2 pi u2), im = r sin(2 pi u2), r = sqrt(-2 ln u1))."""
+    u = SplitMix64(seed).uniforms(2 << n)
+    u1 = 1.0 - u[0::2]
+    u2 = u[1::2]
+    r = np.sqrt(-2.0 * np.log(u1))
+    re = r * np.cos(2.0 * np.pi * u2)
+    im = r * np.sin(2.0 * np.pi * u2)
+    scale = 1.0 / np.sqrt(np.sum(re * re) + np.sum(im * im))
+    return re * scale, im * scale
+
+
 _default = SplitMix64(int(os.environ.get("SPINOZA_SEED", "0x5EED"), 0))
 
 

@@ -54,6 +54,7 @@ def test_adjoint_vs_oracle_param_shift(spz, orc, n, layers):
 
 
 def test_adjoint_16q_vs_oracle(spz, orc):
+    """The full 96-parameter gradient at 16 qubits against oracle parameter shift."""
     print("case adjoint 16q", flush=True)
     from spinoza_amd.autograd import adjoint_gradient
     from spinoza_amd.workloads import hea
@@ -62,9 +63,9 @@ def test_adjoint_16q_vs_oracle(spz, orc):
     ops, thetas, where = hea(n, layers, seed=4)
     s = spz.State(n)
     e, g, _ = adjoint_gradient(s, ops, 1.0, ckpt_every=len(ops) // 3)
-    sub = where[::5]
-    ref = oracle_param_shift(orc, n, ops, sub)
-    got = np.array([g[i] for i in sub])
+    assert abs(e - oracle_energy(orc, n, ops)) < 1e-12
+    ref = oracle_param_shift(orc, n, ops, where)
+    got = np.array([g[i] for i in where])
     assert np.max(np.abs(got - ref)) < 1e-10, np.max(np.abs(got - ref))
 
 

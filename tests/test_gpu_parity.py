@@ -286,6 +286,51 @@ def test_fp32_storage_fidelity(spz, orc):
         assert abs(spz.expect_zz(s, 0, n - 1) - ref.expect_zz(0, n - 1)) < 1e-5
 
 
+def test_inner_mixed_precision(spz, orc):
+    """<a|b> between an FP32 and an FP64 state (either order) against the
+    product of the downloaded vectors, and the FP64 x FP64 case bit-exact
+    against the oracle's fixed-tree dot product."""
+    print("case inner mixed precision", flush=True)
+    from spinoza_amd.executor import run_ops
+    from spinoza_amd.workloads import random_circuit
+    n = 14
+    ops = random_circuit(n, 4, 9)
+    a = spz.State(n, "f32")
+    b = spz.State(n, "f64")
+    run_ops(a, ops)
+    run_ops(b, ops)
+    pa, pb = a.to_numpy(), b.to_numpy()
+    ref = np.vdot(pa, pb)
+    for x, y, r in ((a, b, ref), (b, a, np.conj(ref))):
+        v = spz.inner(x, y)
+        assert abs(v - r) < 1e-12, (v, r)
+    ob = orc.OracleState(n)
+    orc.execute(ob, ops)
+    c = to_gpu(spz, ob)
+    assert spz.inner(c, c).real == ob.norm()
+
+
+def test_state_beyond_2_32_amplitudes(spz):
+    """A 33-qubit FP32 state (2^33 amplitudes, 64 GiB): grids are capped at
+    the dispatch limit and the kernels stride, so the fill, a pair gate on
+    the top qubit (2^32 pairs), a t = 0 gate, a diagonal gate and the norm
+    cover every amplitude."""
+    print("case 33q fp32", flush=True)
+    n = 33
+    s = spz.State(n, "f32")
+    spz.apply(spz.Gate.H, s, n - 1)
+    h = np.float32(2.0 ** -0.5)
+    assert s[0] == (float(h), 0.0) and s[1 << (n - 1)] == (float(h), 0.0)
+    spz.apply(spz.Gate.X, s, 0)
+    spz.apply(spz.Gate.Z, s, n - 1)
+    assert s[0] == (0.0, 0.0) and s[1] == (float(h), 0.0)
+    assert s[(1 << (n - 1)) | 1] == (-float(h), 0.0)
+    assert abs(s.norm() - 2 * float(h) ** 2) < 1e-7
+    re, im = s.download((1 << n) - 4096, 4096)
+    assert not re.any() and not im.any()
+    del s
+
+
 def test_full_size_properties(spz):
     """At the 30-qubit workload size: H on every qubit gives the uniform
     state exactly, X on every qubit maps |0> to |2^n - 1>, norm stays 1."""
