@@ -69,6 +69,8 @@ def parse():
     p.add_argument("--fusion-qubits", type=str, default="32,34",
                    help="FP64 and FP32 qubit counts of the fusion leg")
     p.add_argument("--precision-qubits", type=int, default=28)
+    p.add_argument("--dense-layers", type=int, default=6,
+                   help="layers timed through the dense 5-qubit blocks at 32q/34q")
     p.add_argument("--no-batch", action="store_true", help="skip the batched adjoint leg")
     p.add_argument("--batch-count", type=int, default=1024)
     p.add_argument("--no-qft", action="store_true", help="skip the 20q QFT leg")
@@ -165,14 +167,16 @@ def timed(fn, sync):
     return time.perf_counter() - t0, out
 
 
-def fusion_leg(spz, N, layers, n64, n32, n_prec, seed=3):
+def fusion_leg(spz, N, layers, n64, n32, n_prec, dense_layers, seed=3):
     """Random circuits (the 30q generator) with `layers` layers.
 
     Precision at n_prec qubits: FP64 and FP32 storage through the tile
     executor and the dense 5-qubit blocks against the unfused FP64 plain run:
     fidelity |<psi|psi_ref>|^2, |norm - norm_ref| and |<Z0 Z_n-1> - ref|.
     Throughput at n64 qubits FP64 and n32 qubits FP32 for both executors, with
-    the dense-block kernel's HBM and MFMA fractions."""
+    the dense-block kernel's HBM and MFMA fractions; the dense-block rate is
+    taken on the first `dense_layers` layers of the same circuit (one block
+    pass per 5-qubit run makes the whole circuit take about a minute)."""
     import ctypes
     from spinoza_amd.executor import encode_ops, run_ops
     from spinoza_amd.workloads import random_circuit
@@ -198,11 +202,12 @@ def fusion_leg(spz, N, layers, n64, n32, n_prec, seed=3):
             del s
     del ref
     for n, prec in ((n64, "f64"), (n32, "f32")):
-        ops = random_circuit(n, layers, seed)
-        enc = encode_ops(ops)
+        full = random_circuit(n, layers, seed)
         s = spz.State(n, prec)
-        row = {"gates": len(ops), "state_bytes": (16 if prec == "f64" else 8) << n}
+        row = {"gates": len(full), "state_bytes": (16 if prec == "f64" else 8) << n}
         for name, mode in modes.items():
+            ops = full if name == "tile" else random_circuit(n, min(layers, dense_layers), seed)
+            enc = encode_ops(ops)
             s.set_basis(0)
             N.check(N.lib().spz_profile_enable(1))
             N.check(N.lib().spz_profile_reset())
@@ -212,7 +217,8 @@ def fusion_leg(spz, N, layers, n64, n32, n_prec, seed=3):
             N.check(N.lib().spz_profile_get(cls, ctypes.byref(cnt), ctypes.byref(ms),
                                             ctypes.byref(by)))
             N.check(N.lib().spz_profile_enable(0))
-            r = {"s": round(dt, 3), "gates_per_s": round(len(ops) / dt, 1),
+            r = {"layers": len(ops) // (n + n // 2), "gates": len(ops), "s": round(dt, 3),
+                 "gates_per_s": round(len(ops) / dt, 1),
                  "launches": cnt.value, "kernel_ms": round(ms.value, 1),
                  "hbm_frac": round(by.value / (ms.value * 1e-3) / HBM_PEAK, 4) if ms.value else None,
                  "norm_err": abs(s.norm() - 1.0),
@@ -422,7 +428,8 @@ def main():
         n64, n32 = (int(x) for x in a.fusion_qubits.split(","))
         log(f"fusion leg: {n64}q FP64, {n32}q FP32, {a.fusion_layers} layers; "
             f"precision at {a.precision_qubits}q")
-        result["fusion"] = fusion_leg(spz, N, a.fusion_layers, n64, n32, a.precision_qubits)
+        result["fusion"] = fusion_leg(spz, N, a.fusion_layers, n64, n32, a.precision_qubits,
+                                      a.dense_layers)
 
     if not a.no_batch:
         log(f"batched adjoint leg: {a.batch_count} circuits")
