@@ -19,6 +19,7 @@
 
 #include <algorithm>
 #include <memory>
+#include <mutex>
 #include <vector>
 
 #include "spz_internal.h"
@@ -314,11 +315,6 @@ int launch_group(spz_state_s *psi, spz_state_s *lam, int nq, uint64_t nel, const
     return SPZ_OK;
 }
 
-struct DevBuf {
-    void *p = nullptr;
-    ~DevBuf() { if (p) (void)hipFree(p); }
-};
-
 // Owns a scratch state; not copyable, so it can live in a vector only
 // behind a unique_ptr (a copy would free the state twice).
 struct OwnedState {
@@ -332,6 +328,71 @@ struct OwnedState {
 int make_like(const spz_state_s *s, OwnedState &o)
 {
     return spz_state_create(s->n, s->prec, &o.s);
+}
+
+// Scratch kept between calls (hipFree synchronises the device, and small
+// circuits would otherwise spend most of their time allocating): the two
+// working states of the last shape used and grow-only device buffers.  One
+// lock serialises the adjoint calls that share it.
+struct AdjScratch {
+    std::mutex mu;
+    OwnedState work, lam;
+    void *grad = nullptr, *part = nullptr, *dmax = nullptr;
+    size_t grad_cap = 0, part_cap = 0;
+
+    int states_like(const spz_state_s *s)
+    {
+        if (work.s && (work.s->n != s->n || work.s->prec != s->prec)) {
+            spz_state_destroy(work.s);
+            spz_state_destroy(lam.s);
+            work.s = lam.s = nullptr;
+        }
+        if (!work.s) {
+            SPZ_TRY(make_like(s, work));
+            int rc = make_like(s, lam);
+            if (rc != SPZ_OK) {
+                spz_state_destroy(work.s);
+                work.s = nullptr;
+                return rc;
+            }
+        }
+        return SPZ_OK;
+    }
+
+    static int grow(void **p, size_t *cap, size_t bytes)
+    {
+        if (bytes <= *cap) return SPZ_OK;
+        if (*p) SPZ_HIP(hipFree(*p));
+        *p = nullptr;
+        *cap = 0;
+        if (hipMalloc(p, bytes) != hipSuccess) {
+            (void)hipGetLastError();
+            *p = nullptr;
+            return fail(SPZ_ERR_OOM, "adjoint scratch allocation failed");
+        }
+        *cap = bytes;
+        return SPZ_OK;
+    }
+
+    int buffers(size_t grad_bytes, size_t part_bytes)
+    {
+        SPZ_TRY(grow(&grad, &grad_cap, grad_bytes));
+        SPZ_TRY(grow(&part, &part_cap, part_bytes));
+        if (!dmax) {
+            if (hipMalloc(&dmax, sizeof(unsigned long long)) != hipSuccess) {
+                (void)hipGetLastError();
+                dmax = nullptr;
+                return fail(SPZ_ERR_OOM, "adjoint scratch allocation failed");
+            }
+        }
+        return SPZ_OK;
+    }
+};
+
+AdjScratch &adj_scratch()
+{
+    static AdjScratch *a = new AdjScratch();  // lives until the process ends
+    return *a;
 }
 
 }  // namespace
@@ -379,9 +440,10 @@ int adjoint_grad(spz_state_s *s, const spz_op *ops, uint64_t n_ops, double hx, i
         SPZ_TRY(execute_ops(s, ops, n_ops, nullptr, &eo, nullptr));
     }
     // ---- lambda = H psi, energy --------------------------------------------
-    OwnedState work, lam;
-    SPZ_TRY(make_like(s, work));
-    SPZ_TRY(make_like(s, lam));
+    AdjScratch &sc = adj_scratch();
+    std::lock_guard<std::mutex> lock(sc.mu);
+    SPZ_TRY(sc.states_like(s));
+    OwnedState &work = sc.work, &lam = sc.lam;
     SPZ_TRY(spz_state_copy(work.s, s));
     {
         LaunchTimer tm(SPZ_K_ADJOINT, (double)s->N * (s->prec == SPZ_F64 ? 16.0 : 8.0) * (n + 2));
@@ -399,15 +461,14 @@ int adjoint_grad(spz_state_s *s, const spz_op *ops, uint64_t n_ops, double hx, i
     SPZ_TRY(reduce(s, lam.s, TERM_DOT_RE, 0, 0, 0.0, &e));
     if (energy) *energy = e;
     // ---- reverse sweep -------------------------------------------------------
-    DevBuf dgrad, dpart, dmax;
-    SPZ_HIP(hipMalloc(&dgrad.p, std::max<uint64_t>(n_ops, 1) * sizeof(double)));
-    SPZ_HIP(hipMemsetAsync(dgrad.p, 0, std::max<uint64_t>(n_ops, 1) * sizeof(double), rt().stream));
     const uint64_t per_block = (uint64_t)kT * kGPerThread;
     const uint64_t max_blocks = (s->N / 2 + per_block - 1) / per_block + 1;
-    SPZ_HIP(hipMalloc(&dpart.p, max_blocks * 2 * kGMaxOps * sizeof(double)));
-    SPZ_HIP(hipMalloc(&dmax.p, sizeof(unsigned long long)));
-    SPZ_HIP(hipMemsetAsync(dmax.p, 0, sizeof(unsigned long long), rt().stream));
-    double *gdev = (double *)dgrad.p, *part = (double *)dpart.p;
+    SPZ_TRY(sc.buffers(std::max<uint64_t>(n_ops, 1) * sizeof(double),
+                       max_blocks * 2 * kGMaxOps * sizeof(double)));
+    SPZ_HIP(hipMemsetAsync(sc.grad, 0, std::max<uint64_t>(n_ops, 1) * sizeof(double), rt().stream));
+    SPZ_HIP(hipMemsetAsync(sc.dmax, 0, sizeof(unsigned long long), rt().stream));
+    double *gdev = (double *)sc.grad, *part = (double *)sc.part;
+    void *dmax_p = sc.dmax;
     const double amp = s->prec == SPZ_F64 ? 16.0 : 8.0;
     // run_lo[k]: first index of the run of parameter-free ops that ends at k
     std::vector<uint64_t> run_lo(n_ops);
@@ -497,12 +558,12 @@ int adjoint_grad(spz_state_s *s, const spz_op *ops, uint64_t n_ops, double hx, i
                 hipLaunchKernelGGL((k_maxdiff<double>), dim3(blocks_for(s->N, kT)), dim3(kT), 0,
                                    rt().stream, (const double *)work.s->re,
                                    (const double *)work.s->im, (const double *)ck->re,
-                                   (const double *)ck->im, s->N, (unsigned long long *)dmax.p);
+                                   (const double *)ck->im, s->N, (unsigned long long *)dmax_p);
             else
                 hipLaunchKernelGGL((k_maxdiff<float>), dim3(blocks_for(s->N, kT)), dim3(kT), 0,
                                    rt().stream, (const float *)work.s->re,
                                    (const float *)work.s->im, (const float *)ck->re,
-                                   (const float *)ck->im, s->N, (unsigned long long *)dmax.p);
+                                   (const float *)ck->im, s->N, (unsigned long long *)dmax_p);
             SPZ_HIP(hipGetLastError());
             SPZ_TRY(spz_state_copy(work.s, ck));
             next_ck--;
@@ -511,7 +572,7 @@ int adjoint_grad(spz_state_s *s, const spz_op *ops, uint64_t n_ops, double hx, i
     }
     SPZ_HIP(hipMemcpyAsync(grad, gdev, n_ops * sizeof(double), hipMemcpyDeviceToHost, rt().stream));
     unsigned long long mx = 0;
-    SPZ_HIP(hipMemcpyAsync(&mx, dmax.p, sizeof mx, hipMemcpyDeviceToHost, rt().stream));
+    SPZ_HIP(hipMemcpyAsync(&mx, dmax_p, sizeof mx, hipMemcpyDeviceToHost, rt().stream));
     SPZ_HIP(hipStreamSynchronize(rt().stream));
     if (drift) {
         double d;

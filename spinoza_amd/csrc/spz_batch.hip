@@ -365,18 +365,36 @@ int spz_batch_adjoint(const spz_batch_circuit *circuits, uint64_t count, const s
         if (rc != SPZ_OK) return rc;
     }
     // ---- large circuits: the single-circuit path, one after another -------------
+    // (one state per qubit count, reset to |0...0> before each circuit; sorted
+    // by qubit count so the adjoint's working states are reshaped rarely)
+    std::stable_sort(large.begin(), large.end(),
+                     [&](uint64_t x, uint64_t y) { return circuits[x].n < circuits[y].n; });
+    std::vector<spz_state> st_by_n(64, nullptr);
+    auto release = [&]() {
+        for (spz_state &x : st_by_n)
+            if (x) spz_state_destroy(x);
+    };
     for (uint64_t i : large) {
         const spz_batch_circuit &c = circuits[i];
-        spz_state s = nullptr;
-        SPZ_TRY(spz_state_create(c.n, SPZ_F64, &s));
+        spz_state &s = st_by_n[c.n];
+        int rc = SPZ_OK;
+        if (!s) rc = spz_state_create(c.n, SPZ_F64, &s);
+        else rc = spz_state_set_basis(s, 0);
+        if (rc != SPZ_OK) {
+            release();
+            return rc;
+        }
         auto a = std::chrono::steady_clock::now();
-        int rc = adjoint_grad(s, ops + c.op_offset, (uint64_t)c.n_ops, hx, 0,
-                              grads + c.grad_offset, &energies[i], nullptr);
+        rc = adjoint_grad(s, ops + c.op_offset, (uint64_t)c.n_ops, hx, 0, grads + c.grad_offset,
+                          &energies[i], nullptr);
         auto b = std::chrono::steady_clock::now();
-        spz_state_destroy(s);
-        if (rc != SPZ_OK) return rc;
+        if (rc != SPZ_OK) {
+            release();
+            return rc;
+        }
         lat[i] = std::chrono::duration<double, std::milli>(b - a).count();
     }
+    release();
     auto t1 = std::chrono::steady_clock::now();
     if (stats) {
         std::vector<double> l = lat;

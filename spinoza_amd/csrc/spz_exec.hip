@@ -138,6 +138,24 @@ struct Sweep {
     size_t op_begin, op_count; // range in the program buffer
 };
 
+// One-hot TileGroup bit of an arithmetic kind (0: not a tile kind).
+uint32_t kind_bit(int ak)
+{
+    switch (ak) {
+    case AK_H: return TG_H;
+    case AK_X: return TG_X;
+    case AK_Y: return TG_Y;
+    case AK_RX: return TG_RX;
+    case AK_RY: return TG_RY;
+    case AK_U: return TG_U;
+    case AK_MAT: return TG_MAT;
+    case AK_NEG: return TG_NEG;
+    case AK_PHASE: return TG_PHASE;
+    case AK_RZ: return TG_RZ;
+    default: return 0;
+    }
+}
+
 struct Planner {
     int n, K, LOGW, RB, klo;
     bool strict;
@@ -184,6 +202,7 @@ struct Planner {
         TileOp op;
         memset(&op, 0, sizeof op);
         op.type = TOP_LAYOUT;
+        op.grp = TG_LAYOUT;
         uint32_t used = 0;
         for (int j = 0; j < RB; j++) {
             op.lay.oldreg[j] = (int8_t)oldRS[j];
@@ -356,26 +375,39 @@ struct Planner {
                     if (one) op.mlo |= 1u << b; else op.mlz |= 1u << b;
                 }
             }
+            const bool pred = (op.mro | op.mrz | op.mlo | op.mlz) != 0;
+            const uint32_t kbit = kind_bit(op.ak);
+            if (!kbit) {
+                bad = true;
+                return;
+            }
+            op.grp = TG_GATE | kbit;
             if (g.diag) {
                 op.type = TOP_DIAG;
                 const int b = lbit[g.target];
                 if (b < 0) {
                     op.dsel = 2;
                     op.ebit = 1ull << g.target;
+                    op.grp |= pred ? TG_DLANEP : TG_DLANE;
                 } else if (slot_of(b) >= 0) {
                     op.dsel = 0;
                     op.dpos = slot_of(b);
+                    op.sel = 1u << op.dpos;
+                    op.grp |= pred ? TG_DREGP : TG_DREG;
                 } else {
                     op.dsel = 1;
                     op.dpos = b;
+                    op.grp |= pred ? TG_DLANEP : TG_DLANE;
                 }
             } else {
                 op.type = TOP_PAIR;
-                op.slot = slot_of(lbit[g.target]);
-                if (op.slot < 0) {  // sequence() switches layouts before such a gate
+                const int slot = slot_of(lbit[g.target]);
+                if (slot < 0) {  // sequence() switches layouts before such a gate
                     bad = true;
                     return;
                 }
+                op.sel = 1u << slot;
+                op.grp |= pred ? TG_PAIRP : TG_PAIR;
             }
             prog.push_back(op);
         }

@@ -22,6 +22,22 @@ namespace spz {
 
 enum TileOpType : int32_t { TOP_PAIR = 0, TOP_DIAG = 1, TOP_LAYOUT = 2 };
 
+// One-hot dispatch bits of TileOp::grp: the class of the op, then its
+// arithmetic kind.  The kernel tests them as independent ifs.
+enum TileGroup : uint32_t {
+    TG_LAYOUT = 1u << 0,
+    TG_GATE = 1u << 1,    // any gate (subject to the controls outside the tile)
+    TG_PAIR = 1u << 2,    // 2x2 on a register bit, no controls inside the tile
+    TG_PAIRP = 1u << 3,   // 2x2 on a register bit with controls inside the tile
+    TG_DREG = 1u << 4,    // diagonal, target on a register bit
+    TG_DREGP = 1u << 5,
+    TG_DLANE = 1u << 6,   // diagonal, target on a lane bit or outside the tile
+    TG_DLANEP = 1u << 7,
+    TG_H = 1u << 8, TG_X = 1u << 9, TG_Y = 1u << 10, TG_RX = 1u << 11, TG_RY = 1u << 12,
+    TG_U = 1u << 13, TG_MAT = 1u << 14, TG_NEG = 1u << 15, TG_PHASE = 1u << 16,
+    TG_RZ = 1u << 17,
+};
+
 struct TileLayout {
     int8_t oldreg[8];   // local bit of register-index bit j before ...
     int8_t newreg[8];   // ... and after the change
@@ -33,13 +49,13 @@ struct TileLayout {
 struct TileOp {
     int32_t type;     // TileOpType
     int32_t ak;       // Arith kind
-    int32_t slot;     // PAIR: register-index bit of the target
+    uint32_t sel;     // one-hot register-index bit of the target (PAIR, DIAG with dsel 0)
     int32_t dsel;     // DIAG target: 0 register-index bit dpos, 1 local bit dpos of the
                       // lane part, 2 outside S (global bit ebit)
     int32_t dpos;
     uint32_t mro, mrz;  // controls on register-index bits (must read 1 / 0): uniform
     uint32_t mlo, mlz;  // controls on local bits held by the lane part (per lane)
-    uint32_t pad;
+    uint32_t grp;       // TileGroup bits
     uint64_t eo, ez;    // controls outside S (global index bits), uniform per tile
     uint64_t ebit;      // DIAG with the target outside S: its global bit
     union {

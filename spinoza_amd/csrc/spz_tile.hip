@@ -5,9 +5,20 @@
 // index is wavefront-uniform, so op fields come in through scalar loads).
 // Register arrays are only ever indexed by compile-time constants: the
 // target's register bit selects a template instance, so the amplitudes stay
-// in VGPRs.  Per-gate arithmetic is the same pair_update/diag_update as the
-// plain kernels, so a sweep that keeps program order is bit-identical to
-// one launch per gate.
+// in VGPRs.
+//
+// Two rules keep the amplitudes in one register set for the whole program:
+//  * dispatch is a chain of independent ifs on one-hot bits the host sets
+//    (TileOp::grp, TileOp::sel).  A multi-way branch (a switch, or ifs the
+//    optimiser can prove exclusive) merges ~90 paths into one join, and the
+//    register allocator then keeps the merged amplitudes apart from the
+//    loop-carried ones and copies all of them on every op;
+//  * every update is in place: the 2x2 arithmetic is written as inline
+//    assembly with tied operands, ordered so that each result overwrites an
+//    input whose last use has passed (X and Y are register swaps).
+// The instructions and their order are those of pair_update/diag_update (the
+// reference's per-gate arithmetic, FMA exactly where it writes mul_add), so a
+// sweep that keeps program order is bit-identical to one launch per gate.
 #include "spz_internal.h"
 #include "spz_tile.h"
 
@@ -18,128 +29,340 @@ namespace {
 // LDS bank swizzle for 8-byte elements: XOR the low 4 bits of the element
 // index with two higher nibbles so that power-of-two strides across lanes
 // spread over the banks (a bijection inside every aligned block of 16).
+// swz is linear over XOR, and the lane part and the register part of a local
+// index have disjoint bits, so swz(lane | reg) = swz(lane) ^ swz(reg): the
+// per-lane term is formed once per layout change and the per-register term is
+// wavefront-uniform (scalar), leaving one vector XOR per LDS address.
 __device__ __forceinline__ uint32_t swz(uint32_t l) { return l ^ ((l >> 4) & 15u) ^ ((l >> 8) & 15u); }
 
-// In-register pair update on register bit J.  PRED: some controls sit on
-// register bits; the test on the constant p is wavefront-uniform, and the
-// update is committed with a select so that no control flow splits the
-// register arrays.
+// ---------------------------------------------------------------------------
+// In-place pair arithmetic.  (a, b) = psi0 (re, im), (c, d) = psi1.
+// ---------------------------------------------------------------------------
+
+// H (gates.rs:547-560): a1 = h a, c1 = h c, a = a1 + c1, c = a1 - c1; same for b, d.
+__device__ __forceinline__ void ip_h(double h, double &a, double &b, double &c, double &d)
+{
+    double t0, t1;
+    asm volatile(
+        "v_mul_f64 %0, %6, %2\n\t"
+        "v_mul_f64 %1, %6, %4\n\t"
+        "v_add_f64 %2, %0, %1\n\t"
+        "v_add_f64 %4, %0, -%1\n\t"
+        "v_mul_f64 %0, %6, %3\n\t"
+        "v_mul_f64 %1, %6, %5\n\t"
+        "v_add_f64 %3, %0, %1\n\t"
+        "v_add_f64 %5, %0, -%1"
+        : "=&v"(t0), "=&v"(t1), "+v"(a), "+v"(b), "+v"(c), "+v"(d)
+        : "s"(h));
+}
+
+// X (gates.rs:329-334): swap psi0 and psi1.
+__device__ __forceinline__ void ip_swap(double &x, double &y)
+{
+    double t;
+    asm volatile("v_mov_b64 %2, %0\n\tv_mov_b64 %0, %1\n\tv_mov_b64 %1, %2"
+                 : "+v"(x), "+v"(y), "=&v"(t));
+}
+
+// Y (gates.rs:455-464): (a, b, c, d) <- (d, -c, -b, a); negation is a multiply
+// by -1 (exact, and it flips the sign of zero like -x).
+__device__ __forceinline__ void ip_y(double &a, double &b, double &c, double &d)
+{
+    double t;
+    asm volatile(
+        "v_mov_b64 %4, %0\n\t"
+        "v_mov_b64 %0, %3\n\t"
+        "v_mov_b64 %3, %4\n\t"
+        "v_mov_b64 %4, %1\n\t"
+        "v_mul_f64 %1, -1.0, %2\n\t"
+        "v_mul_f64 %2, -1.0, %4"
+        : "+v"(a), "+v"(b), "+v"(c), "+v"(d), "=&v"(t));
+}
+
+// RX (gates.rs:721-724): a = a cs - d ns, b = b cs + c ns, c = b (-ns) + c cs,
+// d = d cs + a ns, with (cs, ns) = (cos, -sin) of theta/2.
+__device__ __forceinline__ void ip_rx(double cs, double ns, double &a, double &b, double &c,
+                                      double &d)
+{
+    double t0, t1, t2, t3;
+    asm volatile(
+        "v_mul_f64 %0, %4, %8\n\t"   // a cs
+        "v_mul_f64 %1, %7, %9\n\t"   // d ns
+        "v_mul_f64 %2, %7, %8\n\t"   // d cs
+        "v_mul_f64 %3, %4, %9\n\t"   // a ns
+        "v_add_f64 %4, %0, -%1\n\t"
+        "v_add_f64 %7, %2, %3\n\t"
+        "v_mul_f64 %0, %5, %8\n\t"   // b cs
+        "v_mul_f64 %1, %6, %9\n\t"   // c ns
+        "v_mul_f64 %2, %5, -%9\n\t"  // b (-ns)
+        "v_mul_f64 %3, %6, %8\n\t"   // c cs
+        "v_add_f64 %5, %0, %1\n\t"
+        "v_add_f64 %6, %2, %3"
+        : "=&v"(t0), "=&v"(t1), "=&v"(t2), "=&v"(t3), "+v"(a), "+v"(b), "+v"(c), "+v"(d)
+        : "s"(cs), "s"(ns));
+}
+
+// RY (gates.rs:1116-1120): a = a cs - c sn, b = b cs - d sn, c = a sn + c cs,
+// d = b sn + d cs.
+__device__ __forceinline__ void ip_ry(double sn, double cs, double &a, double &b, double &c,
+                                      double &d)
+{
+    double t0, t1, t2, t3;
+    asm volatile(
+        "v_mul_f64 %0, %4, %9\n\t"   // a cs
+        "v_mul_f64 %1, %6, %8\n\t"   // c sn
+        "v_mul_f64 %2, %4, %8\n\t"   // a sn
+        "v_mul_f64 %3, %6, %9\n\t"   // c cs
+        "v_add_f64 %4, %0, -%1\n\t"
+        "v_add_f64 %6, %2, %3\n\t"
+        "v_mul_f64 %0, %5, %9\n\t"   // b cs
+        "v_mul_f64 %1, %7, %8\n\t"   // d sn
+        "v_mul_f64 %2, %5, %8\n\t"   // b sn
+        "v_mul_f64 %3, %7, %9\n\t"   // d cs
+        "v_add_f64 %5, %0, -%1\n\t"
+        "v_add_f64 %7, %2, %3"
+        : "=&v"(t0), "=&v"(t1), "=&v"(t2), "=&v"(t3), "+v"(a), "+v"(b), "+v"(c), "+v"(d)
+        : "s"(sn), "s"(cs));
+}
+
+// U (gates.rs:1248-1266), with (cc, dd, m, n) = (a, b, c, d):
+//   a = fma(g, cc, fma(k, m, -l n))          b = fma(g, dd, fma(k, n, l m))
+//   c = fma(q, cc, fma(-r, dd, fma(s, m, -t n)))
+//   d = fma(q, dd, fma(r, cc, fma(s, n, t m)))
+// c and d are formed first (they need all four inputs), then a and b in place.
+__device__ __forceinline__ void ip_u(const double *k, double &a, double &b, double &c, double &d)
+{
+    double t0, t1, x;
+    asm volatile(
+        "v_mul_f64 %0, -%13, %6\n\t"     // -t n
+        "v_fma_f64 %0, %12, %5, %0\n\t"  // fma(s, m, .)
+        "v_fma_f64 %0, -%11, %4, %0\n\t" // fma(-r, dd, .)
+        "v_fma_f64 %0, %10, %3, %0\n\t"  // c' = fma(q, cc, .)
+        "v_mul_f64 %1, %13, %5\n\t"      // t m
+        "v_fma_f64 %1, %12, %6, %1\n\t"  // fma(s, n, .)
+        "v_fma_f64 %1, %11, %3, %1\n\t"  // fma(r, cc, .)
+        "v_fma_f64 %1, %10, %4, %1\n\t"  // d' = fma(q, dd, .)
+        "v_mul_f64 %2, -%9, %6\n\t"      // -l n
+        "v_fma_f64 %2, %8, %5, %2\n\t"   // fma(k, m, .)
+        "v_fma_f64 %3, %7, %3, %2\n\t"   // a' = fma(g, cc, .)
+        "v_mul_f64 %2, %9, %5\n\t"       // l m
+        "v_fma_f64 %2, %8, %6, %2\n\t"   // fma(k, n, .)
+        "v_fma_f64 %4, %7, %4, %2\n\t"   // b' = fma(g, dd, .)
+        "v_mov_b64 %5, %0\n\t"
+        "v_mov_b64 %6, %1"
+        : "=&v"(t0), "=&v"(t1), "=&v"(x), "+v"(a), "+v"(b), "+v"(c), "+v"(d)
+        : "s"(k[0]), "s"(k[2]), "s"(k[3]), "s"(k[4]), "s"(k[5]), "s"(k[6]), "s"(k[7]));
+}
+
+// RZ multiply in place (gates.rs:940-946): x = fma(x, mr, -y mi),
+// y = fma(x, mi, y mr); y is formed first, while x still holds its input.
+template <bool VCOEF>
+__device__ __forceinline__ void ip_rz(double mr, double mi, double &x, double &y)
+{
+    double t0, t1;
+    if constexpr (VCOEF)
+        asm volatile(
+            "v_mul_f64 %0, %3, %5\n\t"
+            "v_mul_f64 %1, %3, %4\n\t"
+            "v_fma_f64 %3, %2, %5, %1\n\t"
+            "v_fma_f64 %2, %2, %4, -%0"
+            : "=&v"(t0), "=&v"(t1), "+v"(x), "+v"(y)
+            : "v"(mr), "v"(mi));
+    else
+        asm volatile(
+            "v_mul_f64 %0, %3, %5\n\t"
+            "v_mul_f64 %1, %3, %4\n\t"
+            "v_fma_f64 %3, %2, %5, %1\n\t"
+            "v_fma_f64 %2, %2, %4, -%0"
+            : "=&v"(t0), "=&v"(t1), "+v"(x), "+v"(y)
+            : "s"(mr), "s"(mi));
+}
+
+// P multiply in place (gates.rs:835-838): x = fma(x, mr, -y mi),
+// y = fma(y, mr, x mi).
+__device__ __forceinline__ void ip_phase(double mr, double mi, double &x, double &y)
+{
+    double t0, t1;
+    asm volatile(
+        "v_mul_f64 %0, %3, %5\n\t"
+        "v_mul_f64 %1, %2, %5\n\t"
+        "v_fma_f64 %3, %3, %4, %1\n\t"
+        "v_fma_f64 %2, %2, %4, -%0"
+        : "=&v"(t0), "=&v"(t1), "+v"(x), "+v"(y)
+        : "s"(mr), "s"(mi));
+}
+
+// (x, y) <- (x sg, y sg) for a per-lane sg of +1 or -1.
+__device__ __forceinline__ void ip_scale(double sg, double &x, double &y)
+{
+    asm volatile("v_mul_f64 %0, %0, %2\n\tv_mul_f64 %1, %1, %2" : "+v"(x), "+v"(y) : "v"(sg));
+}
+
+template <int AK>
+__device__ __forceinline__ void ip_pair(const double *k, double &a, double &b, double &c, double &d)
+{
+    if constexpr (AK == AK_H) ip_h(k[0], a, b, c, d);
+    else if constexpr (AK == AK_X) { ip_swap(a, c); ip_swap(b, d); }
+    else if constexpr (AK == AK_Y) ip_y(a, b, c, d);
+    else if constexpr (AK == AK_RX) ip_rx(k[0], k[1], a, b, c, d);
+    else if constexpr (AK == AK_RY) ip_ry(k[0], k[1], a, b, c, d);
+    else if constexpr (AK == AK_U) ip_u(k, a, b, c, d);
+    else pair_update(AK, k, a, b, c, d);  // AK_MAT: the dense 2x2 (rare)
+}
+
+// ---------------------------------------------------------------------------
+// Dispatch (one-hot bits set by the host; see TileOp::grp)
+// ---------------------------------------------------------------------------
+
+// Pair update on register bit J.  PRED: controls on register bits (uniform
+// per register) and on lane bits (lane_ok, per lane).
 template <int R, int AK, int J, bool PRED>
 __device__ __forceinline__ void pair_j(double *vr, double *vi, const double *k, uint32_t mro,
-                                       uint32_t mrz)
+                                       uint32_t mrz, bool lane_ok)
 {
 #pragma unroll
     for (int p = 0; p < R; p++) {
         if (p & (1 << J)) continue;
         const int q = p | (1 << J);
         if constexpr (!PRED) {
-            pair_update(AK, k, vr[p], vi[p], vr[q], vi[q]);
+            ip_pair<AK>(k, vr[p], vi[p], vr[q], vi[q]);
         } else {
-            const bool ok = ((uint32_t)p & mro) == mro && !((uint32_t)p & mrz);
-            double a = vr[p], b = vi[p], c = vr[q], d = vi[q];
-            pair_update(AK, k, a, b, c, d);
-            vr[p] = ok ? a : vr[p];
-            vi[p] = ok ? b : vi[p];
-            vr[q] = ok ? c : vr[q];
-            vi[q] = ok ? d : vi[q];
-        }
-    }
-}
-
-template <int R, int RB, int AK>
-__device__ __forceinline__ void pair_slot(int slot, double *vr, double *vi, const double *k,
-                                          uint32_t mro, uint32_t mrz)
-{
-    if ((mro | mrz) == 0) {
-        switch (slot) {
-        case 0: pair_j<R, AK, 0, false>(vr, vi, k, 0, 0); break;
-        case 1: if constexpr (RB > 1) pair_j<R, AK, 1, false>(vr, vi, k, 0, 0); break;
-        case 2: if constexpr (RB > 2) pair_j<R, AK, 2, false>(vr, vi, k, 0, 0); break;
-        case 3: if constexpr (RB > 3) pair_j<R, AK, 3, false>(vr, vi, k, 0, 0); break;
-        default: break;
-        }
-    } else {
-        switch (slot) {
-        case 0: pair_j<R, AK, 0, true>(vr, vi, k, mro, mrz); break;
-        case 1: if constexpr (RB > 1) pair_j<R, AK, 1, true>(vr, vi, k, mro, mrz); break;
-        case 2: if constexpr (RB > 2) pair_j<R, AK, 2, true>(vr, vi, k, mro, mrz); break;
-        case 3: if constexpr (RB > 3) pair_j<R, AK, 3, true>(vr, vi, k, mro, mrz); break;
-        default: break;
-        }
-    }
-}
-
-// Diagonal update of one amplitude; `bit` is the target bit, the two
-// multipliers are values (never selected by address).
-template <int AK>
-__device__ __forceinline__ void diag_one(const double *k, bool bit, double &x, double &y)
-{
-    if constexpr (AK == AK_NEG) {  // z_proc_chunk: negate the |1> amplitude
-        const double nx = -x, ny = -y;
-        x = bit ? nx : x;
-        y = bit ? ny : y;
-    } else if constexpr (AK == AK_PHASE) {  // p_proc_chunk on the |1> amplitude
-        const double zr = x, zi = y;
-        const double nx = __builtin_fma(zr, k[0], -zi * k[1]);
-        const double ny = __builtin_fma(zi, k[0], zr * k[1]);
-        x = bit ? nx : x;
-        y = bit ? ny : y;
-    } else {  // AK_RZ: multiply by diag[bit]
-        const double mr = bit ? k[2] : k[0];
-        const double mi = bit ? k[3] : k[1];
-        const double cr = x, ci = y;
-        x = __builtin_fma(cr, mr, -ci * mi);
-        y = __builtin_fma(cr, mi, ci * mr);
-    }
-}
-
-// DSEL: 0 = target on register bit J (bit known per register), 1 = target on
-// a lane bit (lanebit), 2 = target outside the tile (ebit, uniform).
-template <int R, int AK, int DSEL, int J, bool PRED>
-__device__ __forceinline__ void diag_j(double *vr, double *vi, const double *k, bool lanebit,
-                                       uint32_t mro, uint32_t mrz)
-{
-#pragma unroll
-    for (int p = 0; p < R; p++) {
-        const bool bit = DSEL == 0 ? (((p >> J) & 1) != 0) : lanebit;
-        if constexpr (DSEL == 0 && !PRED && (AK == AK_NEG || AK == AK_PHASE)) {
-            if (!bit) continue;  // |0> amplitudes are untouched: skipped at compile time
-        }
-        if constexpr (!PRED) {
-            diag_one<AK>(k, bit, vr[p], vi[p]);
-        } else {
-            const bool ok = ((uint32_t)p & mro) == mro && !((uint32_t)p & mrz);
-            double x = vr[p], y = vi[p];
-            diag_one<AK>(k, bit, x, y);
-            vr[p] = ok ? x : vr[p];
-            vi[p] = ok ? y : vi[p];
+            if (((uint32_t)p & mro) == mro && !((uint32_t)p & mrz) && lane_ok)
+                ip_pair<AK>(k, vr[p], vi[p], vr[q], vi[q]);
         }
     }
 }
 
 template <int R, int RB, int AK, bool PRED>
-__device__ __forceinline__ void diag_sel(double *vr, double *vi, const double *k, int dsel,
-                                         int dpos, bool lanebit, uint32_t mro, uint32_t mrz)
+__device__ __forceinline__ void pair_sel(uint32_t sel, double *vr, double *vi, const double *k,
+                                         uint32_t mro, uint32_t mrz, bool lane_ok)
 {
-    if (dsel == 0) {
-        switch (dpos) {
-        case 0: diag_j<R, AK, 0, 0, PRED>(vr, vi, k, false, mro, mrz); break;
-        case 1: if constexpr (RB > 1) diag_j<R, AK, 0, 1, PRED>(vr, vi, k, false, mro, mrz); break;
-        case 2: if constexpr (RB > 2) diag_j<R, AK, 0, 2, PRED>(vr, vi, k, false, mro, mrz); break;
-        case 3: if constexpr (RB > 3) diag_j<R, AK, 0, 3, PRED>(vr, vi, k, false, mro, mrz); break;
-        default: break;
-        }
-    } else {
-        diag_j<R, AK, 1, 0, PRED>(vr, vi, k, lanebit, mro, mrz);
+    if (sel & 1u) pair_j<R, AK, 0, PRED>(vr, vi, k, mro, mrz, lane_ok);
+    if constexpr (RB > 1) if (sel & 2u) pair_j<R, AK, 1, PRED>(vr, vi, k, mro, mrz, lane_ok);
+    if constexpr (RB > 2) if (sel & 4u) pair_j<R, AK, 2, PRED>(vr, vi, k, mro, mrz, lane_ok);
+    if constexpr (RB > 3) if (sel & 8u) pair_j<R, AK, 3, PRED>(vr, vi, k, mro, mrz, lane_ok);
+}
+
+template <int R, int RB, bool PRED>
+__device__ __forceinline__ void pair_kinds(uint32_t g, uint32_t sel, double *vr, double *vi,
+                                           const double *k, uint32_t mro, uint32_t mrz,
+                                           bool lane_ok)
+{
+    if (g & TG_H) pair_sel<R, RB, AK_H, PRED>(sel, vr, vi, k, mro, mrz, lane_ok);
+    if (g & TG_X) pair_sel<R, RB, AK_X, PRED>(sel, vr, vi, k, mro, mrz, lane_ok);
+    if (g & TG_Y) pair_sel<R, RB, AK_Y, PRED>(sel, vr, vi, k, mro, mrz, lane_ok);
+    if (g & TG_RX) pair_sel<R, RB, AK_RX, PRED>(sel, vr, vi, k, mro, mrz, lane_ok);
+    if (g & TG_RY) pair_sel<R, RB, AK_RY, PRED>(sel, vr, vi, k, mro, mrz, lane_ok);
+    if (g & TG_U) pair_sel<R, RB, AK_U, PRED>(sel, vr, vi, k, mro, mrz, lane_ok);
+    if (g & TG_MAT) pair_sel<R, RB, AK_MAT, PRED>(sel, vr, vi, k, mro, mrz, lane_ok);
+}
+
+// Diagonal update of one amplitude whose target bit is known at compile
+// time (BITK) or given per lane / per tile (bit, with per-lane multipliers
+// mr, mi for RZ).
+template <int AK>
+__device__ __forceinline__ void diag_known(const double *k, bool bit, double &x, double &y)
+{
+    if constexpr (AK == AK_NEG) {
+        if (bit) { x = -x; y = -y; }
+    } else if constexpr (AK == AK_PHASE) {
+        if (bit) ip_phase(k[0], k[1], x, y);
+    } else {  // AK_RZ: diag[bit]
+        if (bit) ip_rz<false>(k[2], k[3], x, y);
+        else ip_rz<false>(k[0], k[1], x, y);
     }
 }
 
-template <int R, int RB, int AK>
-__device__ __forceinline__ void diag_all(double *vr, double *vi, const double *k, int dsel,
-                                         int dpos, bool lanebit, uint32_t mro, uint32_t mrz)
+// Diagonal target on register bit J.
+template <int R, int AK, int J, bool PRED>
+__device__ __forceinline__ void diag_j(double *vr, double *vi, const double *k, uint32_t mro,
+                                       uint32_t mrz, bool lane_ok)
 {
-    if ((mro | mrz) == 0) diag_sel<R, RB, AK, false>(vr, vi, k, dsel, dpos, lanebit, 0, 0);
-    else diag_sel<R, RB, AK, true>(vr, vi, k, dsel, dpos, lanebit, mro, mrz);
+#pragma unroll
+    for (int p = 0; p < R; p++) {
+        const bool bit = ((p >> J) & 1) != 0;
+        if constexpr (AK == AK_NEG || AK == AK_PHASE)
+            if (!bit) continue;  // |0> amplitudes are untouched
+        if constexpr (!PRED) {
+            diag_known<AK>(k, bit, vr[p], vi[p]);
+        } else {
+            if (((uint32_t)p & mro) == mro && !((uint32_t)p & mrz) && lane_ok)
+                diag_known<AK>(k, bit, vr[p], vi[p]);
+        }
+    }
+}
+
+template <int R, int RB, int AK, bool PRED>
+__device__ __forceinline__ void diag_sel(uint32_t sel, double *vr, double *vi, const double *k,
+                                         uint32_t mro, uint32_t mrz, bool lane_ok)
+{
+    if (sel & 1u) diag_j<R, AK, 0, PRED>(vr, vi, k, mro, mrz, lane_ok);
+    if constexpr (RB > 1) if (sel & 2u) diag_j<R, AK, 1, PRED>(vr, vi, k, mro, mrz, lane_ok);
+    if constexpr (RB > 2) if (sel & 4u) diag_j<R, AK, 2, PRED>(vr, vi, k, mro, mrz, lane_ok);
+    if constexpr (RB > 3) if (sel & 8u) diag_j<R, AK, 3, PRED>(vr, vi, k, mro, mrz, lane_ok);
+}
+
+// Diagonal target on a lane bit or outside the tile: `bit` per lane (or per
+// tile); the RZ multipliers are selected once per op.
+template <int R, int AK, bool PRED>
+__device__ __forceinline__ void diag_lane(double *vr, double *vi, const double *k, bool bit,
+                                          uint32_t mro, uint32_t mrz, bool lane_ok)
+{
+    if constexpr (AK == AK_RZ) {
+        const double mr = bit ? k[2] : k[0];
+        const double mi = bit ? k[3] : k[1];
+#pragma unroll
+        for (int p = 0; p < R; p++) {
+            if constexpr (!PRED) {
+                ip_rz<true>(mr, mi, vr[p], vi[p]);
+            } else {
+                if (((uint32_t)p & mro) == mro && !((uint32_t)p & mrz) && lane_ok)
+                    ip_rz<true>(mr, mi, vr[p], vi[p]);
+            }
+        }
+    } else if constexpr (AK == AK_NEG) {
+        // multiply by -1 or +1 per lane: exact (-x, including the sign of zero)
+        const double sg = bit ? -1.0 : 1.0;
+#pragma unroll
+        for (int p = 0; p < R; p++) {
+            if constexpr (!PRED) {
+                ip_scale(sg, vr[p], vi[p]);
+            } else {
+                if (((uint32_t)p & mro) == mro && !((uint32_t)p & mrz) && lane_ok)
+                    ip_scale(sg, vr[p], vi[p]);
+            }
+        }
+    } else if (bit) {
+#pragma unroll
+        for (int p = 0; p < R; p++) {
+            if constexpr (!PRED) {
+                diag_known<AK>(k, true, vr[p], vi[p]);
+            } else {
+                if (((uint32_t)p & mro) == mro && !((uint32_t)p & mrz) && lane_ok)
+                    diag_known<AK>(k, true, vr[p], vi[p]);
+            }
+        }
+    }
+}
+
+template <int R, int RB, bool PRED>
+__device__ __forceinline__ void diag_reg_kinds(uint32_t g, uint32_t sel, double *vr, double *vi,
+                                               const double *k, uint32_t mro, uint32_t mrz,
+                                               bool lane_ok)
+{
+    if (g & TG_NEG) diag_sel<R, RB, AK_NEG, PRED>(sel, vr, vi, k, mro, mrz, lane_ok);
+    if (g & TG_PHASE) diag_sel<R, RB, AK_PHASE, PRED>(sel, vr, vi, k, mro, mrz, lane_ok);
+    if (g & TG_RZ) diag_sel<R, RB, AK_RZ, PRED>(sel, vr, vi, k, mro, mrz, lane_ok);
+}
+
+template <int R, bool PRED>
+__device__ __forceinline__ void diag_lane_kinds(uint32_t g, double *vr, double *vi,
+                                                const double *k, bool bit, uint32_t mro,
+                                                uint32_t mrz, bool lane_ok)
+{
+    if (g & TG_NEG) diag_lane<R, AK_NEG, PRED>(vr, vi, k, bit, mro, mrz, lane_ok);
+    if (g & TG_PHASE) diag_lane<R, AK_PHASE, PRED>(vr, vi, k, bit, mro, mrz, lane_ok);
+    if (g & TG_RZ) diag_lane<R, AK_RZ, PRED>(vr, vi, k, bit, mro, mrz, lane_ok);
 }
 
 // Register part of the local index of register p under a layout.
@@ -181,68 +404,59 @@ k_tile(T *__restrict__ re, T *__restrict__ im, const TileDesc d, const TileOp *_
     uint32_t lt = tid;
 
     const int nops = d.nops;
-    TileOp cur = ops[0];
     for (int o = 0; o < nops; o++) {
-        // issue the next op's scalar loads before working on this one
-        const TileOp nxt = ops[o + 1 < nops ? o + 1 : o];
-        if (cur.type == TOP_LAYOUT) {
+        // by value: the op's fields come into scalar registers, so selects
+        // between its coefficients are value selects, never per-lane loads
+        const TileOp cur = ops[o];
+        const uint32_t g = cur.grp;
+        if (g & TG_LAYOUT) {
             uint32_t nlt = 0;
 #pragma unroll
             for (int i = 0; i < LOGW; i++)
                 if ((tid >> i) & 1u) nlt |= 1u << cur.lay.newthr[i];
+            // byte offsets into lds: lane term per lane, register term uniform
+            uint32_t so = swz(lt) << 3, sn = swz(nlt) << 3;
+            // (keeps the compiler from factoring the shift back out of the XORs)
+            asm volatile("" : "+v"(so), "+v"(sn));
+            uint32_t ro[R], rn[R];
+#pragma unroll
+            for (int p = 0; p < R; p++) {
+                ro[p] = swz(reg_part<RB>(p, cur.lay.oldreg)) << 3;
+                rn[p] = swz(reg_part<RB>(p, cur.lay.newreg)) << 3;
+            }
+            char *lb = reinterpret_cast<char *>(lds);
             __syncthreads();
 #pragma unroll
-            for (int p = 0; p < R; p++) lds[swz(lt | reg_part<RB>(p, cur.lay.oldreg))] = vr[p];
+            for (int p = 0; p < R; p++) *reinterpret_cast<double *>(lb + (so ^ ro[p])) = vr[p];
             __syncthreads();
 #pragma unroll
-            for (int p = 0; p < R; p++) vr[p] = lds[swz(nlt | reg_part<RB>(p, cur.lay.newreg))];
+            for (int p = 0; p < R; p++) vr[p] = *reinterpret_cast<const double *>(lb + (sn ^ rn[p]));
             __syncthreads();
 #pragma unroll
-            for (int p = 0; p < R; p++) lds[swz(lt | reg_part<RB>(p, cur.lay.oldreg))] = vi[p];
+            for (int p = 0; p < R; p++) *reinterpret_cast<double *>(lb + (so ^ ro[p])) = vi[p];
             __syncthreads();
 #pragma unroll
-            for (int p = 0; p < R; p++) vi[p] = lds[swz(nlt | reg_part<RB>(p, cur.lay.newreg))];
+            for (int p = 0; p < R; p++) vi[p] = *reinterpret_cast<const double *>(lb + (sn ^ rn[p]));
             lt = nlt;
-        } else if ((base & cur.eo) == cur.eo && !(base & cur.ez)) {
-            // (a control outside the tile that is not met skips the op)
-            const uint32_t mlo = cur.mlo, mlz = cur.mlz;
-            const bool lane_ok = ((mlo | mlz) == 0) || ((lt & mlo) == mlo && !(lt & mlz));
-            if (lane_ok) {
-                const uint32_t mro = cur.mro, mrz = cur.mrz;
-                const int ak = cur.ak;
-                const double *k = cur.c;
-                if (cur.type == TOP_PAIR) {
-                    const int slot = cur.slot;
-                    switch (ak) {
-                    case AK_H: pair_slot<R, RB, AK_H>(slot, vr, vi, k, mro, mrz); break;
-                    case AK_X: pair_slot<R, RB, AK_X>(slot, vr, vi, k, mro, mrz); break;
-                    case AK_Y: pair_slot<R, RB, AK_Y>(slot, vr, vi, k, mro, mrz); break;
-                    case AK_RX: pair_slot<R, RB, AK_RX>(slot, vr, vi, k, mro, mrz); break;
-                    case AK_RY: pair_slot<R, RB, AK_RY>(slot, vr, vi, k, mro, mrz); break;
-                    case AK_U: pair_slot<R, RB, AK_U>(slot, vr, vi, k, mro, mrz); break;
-                    case AK_MAT: pair_slot<R, RB, AK_MAT>(slot, vr, vi, k, mro, mrz); break;
-                    default: break;
-                    }
-                } else {
-                    int dsel = cur.dsel;
-                    const int dpos = cur.dpos;
-                    bool lanebit;
-                    if (dsel == 2) {  // target outside the tile: the same bit for all
-                        lanebit = (base & cur.ebit) != 0;
-                        dsel = 1;
-                    } else {
-                        lanebit = ((lt >> dpos) & 1u) != 0;
-                    }
-                    switch (ak) {
-                    case AK_NEG: diag_all<R, RB, AK_NEG>(vr, vi, k, dsel, dpos, lanebit, mro, mrz); break;
-                    case AK_PHASE: diag_all<R, RB, AK_PHASE>(vr, vi, k, dsel, dpos, lanebit, mro, mrz); break;
-                    case AK_RZ: diag_all<R, RB, AK_RZ>(vr, vi, k, dsel, dpos, lanebit, mro, mrz); break;
-                    default: break;
-                    }
-                }
+        }
+        // (a control outside the tile that is not met skips the op)
+        if ((g & TG_GATE) && (base & cur.eo) == cur.eo && !(base & cur.ez)) {
+            const uint32_t sel = cur.sel, mro = cur.mro, mrz = cur.mrz;
+            // controls on lane bits: a per-lane predicate inside the predicated paths
+            const bool lane_ok = (lt & cur.mlo) == cur.mlo && !(lt & cur.mlz);
+            const double *k = cur.c;
+            if (g & TG_PAIR) pair_kinds<R, RB, false>(g, sel, vr, vi, k, 0, 0, true);
+            if (g & TG_PAIRP) pair_kinds<R, RB, true>(g, sel, vr, vi, k, mro, mrz, lane_ok);
+            if (g & TG_DREG) diag_reg_kinds<R, RB, false>(g, sel, vr, vi, k, 0, 0, true);
+            if (g & TG_DREGP) diag_reg_kinds<R, RB, true>(g, sel, vr, vi, k, mro, mrz, lane_ok);
+            if (g & (TG_DLANE | TG_DLANEP)) {
+                // target on a lane bit (dsel 1) or outside the tile (dsel 2)
+                const bool bit = cur.dsel == 2 ? (base & cur.ebit) != 0
+                                               : ((lt >> cur.dpos) & 1u) != 0;
+                if (g & TG_DLANE) diag_lane_kinds<R, false>(g, vr, vi, k, bit, 0, 0, true);
+                if (g & TG_DLANEP) diag_lane_kinds<R, true>(g, vr, vi, k, bit, mro, mrz, lane_ok);
             }
         }
-        cur = nxt;
     }
 
 #pragma unroll
