@@ -239,19 +239,26 @@ k_ising_apply(const T *__restrict__ re, const T *__restrict__ im, T *__restrict_
     }
 }
 
-// max |a - b| over the state, into out[0] via an integer atomic max on the
-// (non-negative) double bit patterns.
+// max |a - b| over the state, into out[0]: a grid-stride maximum per thread,
+// a wavefront shuffle and an LDS step per block, then one integer atomic max
+// per block on the (non-negative) double bit patterns.
 template <typename T>
 __global__ void __launch_bounds__(kT)
 k_maxdiff(const T *__restrict__ ar, const T *__restrict__ ai, const T *__restrict__ br,
           const T *__restrict__ bi, uint64_t N, unsigned long long *__restrict__ out)
 {
+    __shared__ double W[kT / 64];
     const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
     double m = 0.0;
     for (uint64_t j = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; j < N; j += stride)
         m = fmax(m, fmax(fabs((double)ar[j] - (double)br[j]), fabs((double)ai[j] - (double)bi[j])));
     for (int off = 32; off >= 1; off >>= 1) m = fmax(m, __shfl_xor(m, off));
-    if ((threadIdx.x & 63) == 0) atomicMax(out, (unsigned long long)__double_as_longlong(m));
+    if ((threadIdx.x & 63) == 0) W[threadIdx.x >> 6] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int w = 1; w < kT / 64; w++) m = fmax(m, W[w]);
+        atomicMax(out, (unsigned long long)__double_as_longlong(m));
+    }
 }
 
 // Blocks of kT threads covering `units` (per_block units each), capped at the
@@ -262,6 +269,13 @@ inline unsigned blocks_for(uint64_t units, uint64_t per_block)
     const uint64_t cap = 0xFFFFFFFFull / kT;
     if (g > cap) g = cap;
     return (unsigned)(g == 0 ? 1 : g);
+}
+
+// Blocks of the drift check: enough to fill the GPU, each striding over the state.
+unsigned maxdiff_grid(uint64_t N)
+{
+    const uint64_t cap = (uint64_t)std::max(rt().cu_count, 1) * 8;
+    return (unsigned)std::max<uint64_t>(1, std::min<uint64_t>(cap, (N + kT - 1) / kT));
 }
 
 bool has_param(int kind)
@@ -555,12 +569,12 @@ int adjoint_grad(spz_state_s *s, const spz_op *ops, uint64_t n_ops, double hx, i
         if (next_ck >= 0 && lo >= 1 && ckpt_at[next_ck] == lo - 1) {
             spz_state_s *ck = ckpts[next_ck]->s;
             if (s->prec == SPZ_F64)
-                hipLaunchKernelGGL((k_maxdiff<double>), dim3(blocks_for(s->N, kT)), dim3(kT), 0,
+                hipLaunchKernelGGL((k_maxdiff<double>), dim3(maxdiff_grid(s->N)), dim3(kT), 0,
                                    rt().stream, (const double *)work.s->re,
                                    (const double *)work.s->im, (const double *)ck->re,
                                    (const double *)ck->im, s->N, (unsigned long long *)dmax_p);
             else
-                hipLaunchKernelGGL((k_maxdiff<float>), dim3(blocks_for(s->N, kT)), dim3(kT), 0,
+                hipLaunchKernelGGL((k_maxdiff<float>), dim3(maxdiff_grid(s->N)), dim3(kT), 0,
                                    rt().stream, (const float *)work.s->re,
                                    (const float *)work.s->im, (const float *)ck->re,
                                    (const float *)ck->im, s->N, (unsigned long long *)dmax_p);

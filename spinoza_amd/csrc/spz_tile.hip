@@ -227,7 +227,7 @@ __device__ __forceinline__ void pair_j(double *vr, double *vi, const double *k, 
         if constexpr (!PRED) {
             ip_pair<AK>(k, vr[p], vi[p], vr[q], vi[q]);
         } else {
-            if (((uint32_t)p & mro) == mro && !((uint32_t)p & mrz) && lane_ok)
+            if (((uint32_t)p & mro) == mro && !((uint32_t)p & mrz))
                 ip_pair<AK>(k, vr[p], vi[p], vr[q], vi[q]);
         }
     }
@@ -286,7 +286,7 @@ __device__ __forceinline__ void diag_j(double *vr, double *vi, const double *k, 
         if constexpr (!PRED) {
             diag_known<AK>(k, bit, vr[p], vi[p]);
         } else {
-            if (((uint32_t)p & mro) == mro && !((uint32_t)p & mrz) && lane_ok)
+            if (((uint32_t)p & mro) == mro && !((uint32_t)p & mrz))
                 diag_known<AK>(k, bit, vr[p], vi[p]);
         }
     }
@@ -316,7 +316,7 @@ __device__ __forceinline__ void diag_lane(double *vr, double *vi, const double *
             if constexpr (!PRED) {
                 ip_rz<true>(mr, mi, vr[p], vi[p]);
             } else {
-                if (((uint32_t)p & mro) == mro && !((uint32_t)p & mrz) && lane_ok)
+                if (((uint32_t)p & mro) == mro && !((uint32_t)p & mrz))
                     ip_rz<true>(mr, mi, vr[p], vi[p]);
             }
         }
@@ -328,7 +328,7 @@ __device__ __forceinline__ void diag_lane(double *vr, double *vi, const double *
             if constexpr (!PRED) {
                 ip_scale(sg, vr[p], vi[p]);
             } else {
-                if (((uint32_t)p & mro) == mro && !((uint32_t)p & mrz) && lane_ok)
+                if (((uint32_t)p & mro) == mro && !((uint32_t)p & mrz))
                     ip_scale(sg, vr[p], vi[p]);
             }
         }
@@ -338,7 +338,7 @@ __device__ __forceinline__ void diag_lane(double *vr, double *vi, const double *
             if constexpr (!PRED) {
                 diag_known<AK>(k, true, vr[p], vi[p]);
             } else {
-                if (((uint32_t)p & mro) == mro && !((uint32_t)p & mrz) && lane_ok)
+                if (((uint32_t)p & mro) == mro && !((uint32_t)p & mrz))
                     diag_known<AK>(k, true, vr[p], vi[p]);
             }
         }
@@ -446,15 +446,20 @@ k_tile(T *__restrict__ re, T *__restrict__ im, const TileDesc d, const TileOp *_
             const bool lane_ok = (lt & cur.mlo) == cur.mlo && !(lt & cur.mlz);
             const double *k = cur.c;
             if (g & TG_PAIR) pair_kinds<R, RB, false>(g, sel, vr, vi, k, 0, 0, true);
-            if (g & TG_PAIRP) pair_kinds<R, RB, true>(g, sel, vr, vi, k, mro, mrz, lane_ok);
+            // predicated ops: the lane part of the controls is one divergent
+            // branch around the op, the register part a uniform test per register
+            if (g & TG_PAIRP)
+                if (lane_ok) pair_kinds<R, RB, true>(g, sel, vr, vi, k, mro, mrz, true);
             if (g & TG_DREG) diag_reg_kinds<R, RB, false>(g, sel, vr, vi, k, 0, 0, true);
-            if (g & TG_DREGP) diag_reg_kinds<R, RB, true>(g, sel, vr, vi, k, mro, mrz, lane_ok);
+            if (g & TG_DREGP)
+                if (lane_ok) diag_reg_kinds<R, RB, true>(g, sel, vr, vi, k, mro, mrz, true);
             if (g & (TG_DLANE | TG_DLANEP)) {
                 // target on a lane bit (dsel 1) or outside the tile (dsel 2)
                 const bool bit = cur.dsel == 2 ? (base & cur.ebit) != 0
                                                : ((lt >> cur.dpos) & 1u) != 0;
                 if (g & TG_DLANE) diag_lane_kinds<R, false>(g, vr, vi, k, bit, 0, 0, true);
-                if (g & TG_DLANEP) diag_lane_kinds<R, true>(g, vr, vi, k, bit, mro, mrz, lane_ok);
+                if (g & TG_DLANEP)
+                    if (lane_ok) diag_lane_kinds<R, true>(g, vr, vi, k, bit, mro, mrz, true);
             }
         }
     }
