@@ -213,6 +213,20 @@ int spz_sample(spz_state s, const double *uniforms, uint64_t k, uint64_t *out);
 int spz_adjoint_grad(spz_state s, const spz_op *ops, uint64_t n_ops, double hx,
                      int ckpt_every, double *grad, double *energy, double *drift);
 
+/* ---- executor planning statistics (no counterpart in the reference) ------
+ * Host-only: how the tile executor (SPZ_EXEC_AUTO / SPZ_EXEC_STRICT) would
+ * split `ops` for an n-qubit state, without touching a device.  For tuning and
+ * tests of the scheduler. */
+typedef struct spz_plan_info {
+    uint64_t sweeps;        /* tile sweeps (one read + write of the state each) */
+    uint64_t tile_gates;    /* gates applied inside sweeps                       */
+    uint64_t layouts;       /* register-layout changes inside sweeps             */
+    uint64_t plain_gates;   /* gates left to one-kernel-per-gate passes          */
+    uint64_t reserved[4];
+} spz_plan_info;
+int spz_plan_stats(int n, int precision, const spz_op *ops, uint64_t n_ops,
+                   const spz_exec_opts *opts, spz_plan_info *out);
+
 /* ---- batched differentiation (no counterpart in the reference) ---------- */
 typedef struct spz_batch_circuit {
     int32_t n;              /* qubits (1..30); every circuit starts in |0...0> */

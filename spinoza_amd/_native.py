@@ -68,6 +68,12 @@ class BatchOpts_t(ctypes.Structure):
                 ("reserved", ctypes.c_int32 * 6)]
 
 
+class PlanInfo_t(ctypes.Structure):
+    _fields_ = [("sweeps", ctypes.c_uint64), ("tile_gates", ctypes.c_uint64),
+                ("layouts", ctypes.c_uint64), ("plain_gates", ctypes.c_uint64),
+                ("reserved", ctypes.c_uint64 * 4)]
+
+
 class BatchStats_t(ctypes.Structure):
     _fields_ = [("total_ms", ctypes.c_double), ("p50_ms", ctypes.c_double),
                 ("p99_ms", ctypes.c_double), ("n_small", ctypes.c_uint64),
@@ -121,6 +127,8 @@ SIGNATURES = {
                                ctypes.POINTER(BatchStats_t)]),
     "spz_batch_autotune": (_i, [ctypes.POINTER(BatchCircuit_t), _u64, ctypes.POINTER(Op_t),
                                 ctypes.POINTER(_i)]),
+    "spz_plan_stats": (_i, [_i, _i, ctypes.POINTER(Op_t), _u64, ctypes.POINTER(ExecOpts_t),
+                           ctypes.POINTER(PlanInfo_t)]),
     "spz_profile_enable": (_i, [_i]),
     "spz_profile_reset": (_i, []),
     "spz_profile_get": (_i, [_i, _u64p, _dp, _dp]),

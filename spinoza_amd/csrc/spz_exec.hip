@@ -158,6 +158,7 @@ uint32_t kind_bit(int ak)
 
 struct Planner {
     int n, K, LOGW, RB, klo;
+    int improve = 0;  // local-search passes over the tile set (AUTO only)
     bool strict;
     bool bad = false;
     std::vector<TileOp> prog;
@@ -194,7 +195,69 @@ struct Planner {
             }
         }
         for (int q = 0; q < n && __builtin_popcountll(smask) < K; q++) smask |= 1ull << q;
+        if (!strict && improve > 0 && n > K) improve_set(run, rem, smask, order);
         return order;
+    }
+
+    // Gates of `rem` that one sweep over the fixed tile set `smask` can take
+    // (program order, skipping past blocked gates); stops once every tile
+    // qubit is blocked.  With `out`, also lists them.
+    size_t take_fixed(const std::vector<SG> &run, const std::vector<int> &rem, uint64_t smask,
+                      std::vector<int> *out) const
+    {
+        uint64_t bnd = 0, bd = 0;
+        size_t cnt = 0;
+        for (int idx : rem) {
+            const SG &g = run[idx];
+            if (!(g.nd & (bnd | bd)) && !(g.dm & bnd) && !(g.nd & ~smask)) {
+                cnt++;
+                if (out) out->push_back(idx);
+            } else {
+                bnd |= g.nd;
+                bd |= g.dm;
+                if ((bnd & smask) == smask) break;
+            }
+        }
+        return cnt;
+    }
+
+    // Local search over the tile set: swap a free tile qubit for a qubit that
+    // the next gates target, keep the first swap that lets the sweep take more
+    // gates, repeat.  The low klo qubits stay (they make the tile rows contiguous).
+    void improve_set(const std::vector<SG> &run, const std::vector<int> &rem, uint64_t &smask,
+                     std::vector<int> &order) const
+    {
+        const uint64_t fixed = (klo >= 64) ? ~0ull : ((1ull << klo) - 1);
+        uint64_t cand = 0;  // non-diagonal targets among the next gates
+        size_t seen = 0;
+        for (int idx : rem) {
+            cand |= run[idx].nd;
+            if (++seen >= 4 * (size_t)n * 8) break;
+        }
+        cand &= ~fixed;
+        size_t best = order.size();
+        bool moved = false;
+        for (int pass = 0; pass < improve; pass++) {
+            bool better = false;
+            for (int a = 0; a < n && !better; a++) {
+                if (!((smask >> a) & 1) || ((fixed >> a) & 1)) continue;
+                for (int b = 0; b < n && !better; b++) {
+                    if (((smask >> b) & 1) || !((cand >> b) & 1)) continue;
+                    const uint64_t m2 = (smask & ~(1ull << a)) | (1ull << b);
+                    const size_t c2 = take_fixed(run, rem, m2, nullptr);
+                    if (c2 > best) {
+                        best = c2;
+                        smask = m2;
+                        better = moved = true;
+                    }
+                }
+            }
+            if (!better) break;
+        }
+        if (moved) {
+            order.clear();
+            take_fixed(run, rem, smask, &order);
+        }
     }
 
     void emit_layout(const std::vector<int> &oldRS, const std::vector<int> &RSv)
@@ -459,6 +522,7 @@ int run_fused(spz_state_s *s, const std::vector<SG> &run, int K, bool strict)
     pl.LOGW = K - 4;
     pl.RB = 4;
     pl.klo = std::min(env_int("SPZ_TILE_KLO", 4), K);
+    pl.improve = env_int("SPZ_TILE_SEARCH", 16);
     pl.strict = strict;
     pl.plan(run);
     if (pl.bad) return fail(SPZ_ERR_ARG, "tile planner produced a gate outside the register layout");
@@ -634,6 +698,76 @@ int execute_plain(spz_state_s *s, const spz_op *ops, uint64_t n_ops, spz_tracker
     return SPZ_OK;
 }
 
+// The tile width and low-qubit count an execution uses (options, then the
+// SPZ_TILE_K / SPZ_TILE_KLO environment, then 12 and 4).
+int tile_width(const spz_exec_opts *opts, int n, int *K_out)
+{
+    int K = (opts && opts->tile_qubits) ? opts->tile_qubits : env_int("SPZ_TILE_K", 12);
+    if (K < 10 || K > 13) return fail(SPZ_ERR_ARG, "tile_qubits must be in [10, 13]");
+    if (n < K + 1) K = n - 1;
+    *K_out = K;
+    return SPZ_OK;
+}
+
+// Planning only (no device work): how the tile executor would split `ops`.
+int plan_info(int n, int prec, const spz_op *ops, uint64_t n_ops, const spz_exec_opts *opts,
+              spz_plan_info *out)
+{
+    if (!out || (n_ops && !ops)) return fail(SPZ_ERR_ARG, "null argument");
+    if (n < 1 || n > 40) return fail(SPZ_ERR_QUBIT, "qubit count must be in [1, 40]");
+    memset(out, 0, sizeof *out);
+    const int mode = opts ? opts->mode : SPZ_EXEC_AUTO;
+    if (mode != SPZ_EXEC_AUTO && mode != SPZ_EXEC_STRICT)
+        return fail(SPZ_ERR_ARG, "plan statistics cover the tile executors (AUTO, STRICT)");
+    int K;
+    SPZ_TRY(tile_width(opts, n, &K));
+    spz_state_s st;  // shape only: lowering reads n and the precision
+    memset(&st, 0, sizeof st);
+    st.n = n;
+    st.prec = prec;
+    st.N = 1ull << n;
+    uint64_t i = 0;
+    while (i < n_ops) {
+        if (is_barrier(ops[i]) || K < 10) {
+            out->plain_gates++;
+            i++;
+            continue;
+        }
+        std::vector<SG> run;
+        uint64_t j = i;
+        for (; j < n_ops && !is_barrier(ops[j]); j++) {
+            SG g;
+            int rc = lower(&st, ops[j], nullptr, (int)j, &g);
+            if (rc == -1) continue;
+            if (rc != SPZ_OK) return rc;
+            run.push_back(g);
+        }
+        Planner pl;
+        pl.n = n;
+        pl.K = K;
+        pl.LOGW = K - 4;
+        pl.RB = 4;
+        pl.klo = std::min(env_int("SPZ_TILE_KLO", 4), K);
+        pl.improve = env_int("SPZ_TILE_SEARCH", 16);
+        pl.strict = mode == SPZ_EXEC_STRICT;
+        pl.plan(run);
+        if (pl.bad) return fail(SPZ_ERR_ARG, "tile planner produced a gate outside the register layout");
+        for (const Sweep &sw : pl.sweeps) {
+            if (!sw.tile) {
+                out->plain_gates++;
+                continue;
+            }
+            out->sweeps++;
+            for (size_t o = sw.op_begin; o < sw.op_begin + sw.op_count; o++) {
+                if (pl.prog[o].type == TOP_LAYOUT) out->layouts++;
+                else out->tile_gates++;
+            }
+        }
+        i = j;
+    }
+    return SPZ_OK;
+}
+
 int execute_ops(spz_state_s *s, const spz_op *ops, uint64_t n_ops, spz_tracker *trk,
                 const spz_exec_opts *opts, uint8_t *outcomes)
 {
@@ -643,9 +777,8 @@ int execute_ops(spz_state_s *s, const spz_op *ops, uint64_t n_ops, spz_tracker *
         return fail(SPZ_ERR_ARG, "unknown execution mode");
     const bool dense5 = mode == SPZ_EXEC_DENSE5;
     if (dense5 && s->n < 12) return execute_plain(s, ops, n_ops, trk, outcomes);
-    int K = (opts && opts->tile_qubits) ? opts->tile_qubits : env_int("SPZ_TILE_K", 12);
-    if (K < 10 || K > 13) return fail(SPZ_ERR_ARG, "tile_qubits must be in [10, 13]");
-    if (s->n < K + 1) K = s->n - 1;
+    int K;
+    SPZ_TRY(tile_width(opts, s->n, &K));
     if (K < 10) return execute_plain(s, ops, n_ops, trk, outcomes);
     const bool strict = mode == SPZ_EXEC_STRICT;
     if (outcomes)
@@ -680,3 +813,10 @@ int execute_ops(spz_state_s *s, const spz_op *ops, uint64_t n_ops, spz_tracker *
 }
 
 }  // namespace spz
+
+extern "C" int spz_plan_stats(int n, int precision, const spz_op *ops, uint64_t n_ops,
+                              const spz_exec_opts *opts, spz_plan_info *out)
+{
+    return spz::plan_info(n, precision, ops, n_ops, opts, out);
+}
+

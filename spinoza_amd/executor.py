@@ -52,3 +52,19 @@ def run_ops(state, ops, mode=N.EXEC_AUTO, tracker=None, tile_qubits=0, encoded=N
                                 outcomes))
     del keep
     return [int(outcomes[i]) for i in range(n)]
+
+
+def plan_stats(n, ops, mode=N.EXEC_AUTO, tile_qubits=0, precision="f64"):
+    """How the tile executor splits `ops` for an n-qubit state (host only, no
+    device work): dict with sweeps, tile_gates, layouts, plain_gates."""
+    arr, keep = encode_ops(ops)
+    opts = N.ExecOpts_t()
+    opts.mode = mode
+    opts.tile_qubits = tile_qubits
+    info = N.PlanInfo_t()
+    prec = N.F64 if precision == "f64" else N.F32
+    N.check(N.lib().spz_plan_stats(int(n), prec, arr, len(ops), ctypes.byref(opts),
+                                   ctypes.byref(info)))
+    del keep
+    return {"sweeps": info.sweeps, "tile_gates": info.tile_gates, "layouts": info.layouts,
+            "plain_gates": info.plain_gates}

@@ -50,6 +50,9 @@ int main(void) {
   printf("%zu %zu %zu %zu %zu %zu %zu\n", offsetof(spz_op, theta), offsetof(spz_op, t1),
          offsetof(spz_op, ones), offsetof(spz_op, u), offsetof(spz_op, m), offsetof(spz_op, u_re),
          offsetof(spz_op, u_im));
+  printf("%zu %zu %zu %zu %zu %zu\n", sizeof(spz_batch_circuit), sizeof(spz_batch_opts),
+         sizeof(spz_batch_stats), sizeof(spz_plan_info), offsetof(spz_batch_stats, n_small),
+         offsetof(spz_plan_info, plain_gates));
   return 0;
 }
 ''')
@@ -63,6 +66,10 @@ int main(void) {
                      ctypes.sizeof(N.ExecOpts_t)]
     assert offs == [N.Op_t.theta.offset, N.Op_t.t1.offset, N.Op_t.ones.offset, N.Op_t.u.offset,
                     N.Op_t.m.offset, N.Op_t.u_re.offset, N.Op_t.u_im.offset]
+    more = list(map(int, lines[2].split()))
+    assert more == [ctypes.sizeof(N.BatchCircuit_t), ctypes.sizeof(N.BatchOpts_t),
+                    ctypes.sizeof(N.BatchStats_t), ctypes.sizeof(N.PlanInfo_t),
+                    N.BatchStats_t.n_small.offset, N.PlanInfo_t.plain_gates.offset]
 
 
 def test_enum_values_match_header():
