@@ -138,7 +138,8 @@ struct Sweep {
     size_t op_begin, op_count; // range in the program buffer
 };
 
-// One-hot TileGroup bit of an arithmetic kind (0: not a tile kind).
+// One-hot TileGroup bit of an arithmetic kind (0: not a tile kind; the gates
+// lowered by lower() never carry AK_MAT).
 uint32_t kind_bit(int ak)
 {
     switch (ak) {
@@ -148,7 +149,6 @@ uint32_t kind_bit(int ak)
     case AK_RX: return TG_RX;
     case AK_RY: return TG_RY;
     case AK_U: return TG_U;
-    case AK_MAT: return TG_MAT;
     case AK_NEG: return TG_NEG;
     case AK_PHASE: return TG_PHASE;
     case AK_RZ: return TG_RZ;
@@ -423,6 +423,7 @@ struct Planner {
                     if (RS[j] == b) return j;
                 return -1;
             };
+            uint32_t mro = 0, mrz = 0;  // controls on register-index bits (read 1 / 0)
             for (int q = 0; q < n; q++) {
                 const bool one = (g.ones >> q) & 1, zero = (g.zeros >> q) & 1;
                 if (!one && !zero) continue;
@@ -433,12 +434,14 @@ struct Planner {
                 }
                 const int j = slot_of(b);
                 if (j >= 0) {
-                    if (one) op.mro |= 1u << j; else op.mrz |= 1u << j;
+                    if (one) mro |= 1u << j; else mrz |= 1u << j;
                 } else {
                     if (one) op.mlo |= 1u << b; else op.mlz |= 1u << b;
                 }
             }
-            const bool pred = (op.mro | op.mrz | op.mlo | op.mlz) != 0;
+            for (uint32_t p = 0; p < (1u << RB); p++)
+                if ((p & mro) == mro && !(p & mrz)) op.pm |= 1u << p;
+            const bool pred = (mro | mrz | op.mlo | op.mlz) != 0;
             const uint32_t kbit = kind_bit(op.ak);
             if (!kbit) {
                 bad = true;

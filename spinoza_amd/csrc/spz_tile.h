@@ -34,7 +34,7 @@ enum TileGroup : uint32_t {
     TG_DLANE = 1u << 6,   // diagonal, target on a lane bit or outside the tile
     TG_DLANEP = 1u << 7,
     TG_H = 1u << 8, TG_X = 1u << 9, TG_Y = 1u << 10, TG_RX = 1u << 11, TG_RY = 1u << 12,
-    TG_U = 1u << 13, TG_MAT = 1u << 14, TG_NEG = 1u << 15, TG_PHASE = 1u << 16,
+    TG_U = 1u << 13, TG_NEG = 1u << 15, TG_PHASE = 1u << 16,
     TG_RZ = 1u << 17,
 };
 
@@ -53,7 +53,8 @@ struct TileOp {
     int32_t dsel;     // DIAG target: 0 register-index bit dpos, 1 local bit dpos of the
                       // lane part, 2 outside S (global bit ebit)
     int32_t dpos;
-    uint32_t mro, mrz;  // controls on register-index bits (must read 1 / 0): uniform
+    uint32_t pm;        // bit p: register p meets the controls on register-index bits
+    uint32_t pad;
     uint32_t mlo, mlz;  // controls on local bits held by the lane part (per lane)
     uint32_t grp;       // TileGroup bits
     uint64_t eo, ez;    // controls outside S (global index bits), uniform per tile

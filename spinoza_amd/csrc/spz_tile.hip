@@ -206,8 +206,7 @@ __device__ __forceinline__ void ip_pair(const double *k, double &a, double &b, d
     else if constexpr (AK == AK_Y) ip_y(a, b, c, d);
     else if constexpr (AK == AK_RX) ip_rx(k[0], k[1], a, b, c, d);
     else if constexpr (AK == AK_RY) ip_ry(k[0], k[1], a, b, c, d);
-    else if constexpr (AK == AK_U) ip_u(k, a, b, c, d);
-    else pair_update(AK, k, a, b, c, d);  // AK_MAT: the dense 2x2 (rare)
+    else ip_u(k, a, b, c, d);
 }
 
 // ---------------------------------------------------------------------------
@@ -217,8 +216,7 @@ __device__ __forceinline__ void ip_pair(const double *k, double &a, double &b, d
 // Pair update on register bit J.  PRED: controls on register bits (uniform
 // per register) and on lane bits (lane_ok, per lane).
 template <int R, int AK, int J, bool PRED>
-__device__ __forceinline__ void pair_j(double *vr, double *vi, const double *k, uint32_t mro,
-                                       uint32_t mrz, bool lane_ok)
+__device__ __forceinline__ void pair_j(double *vr, double *vi, const double *k, uint32_t pm, bool lane_ok)
 {
 #pragma unroll
     for (int p = 0; p < R; p++) {
@@ -227,7 +225,7 @@ __device__ __forceinline__ void pair_j(double *vr, double *vi, const double *k, 
         if constexpr (!PRED) {
             ip_pair<AK>(k, vr[p], vi[p], vr[q], vi[q]);
         } else {
-            if (((uint32_t)p & mro) == mro && !((uint32_t)p & mrz))
+            if ((pm >> p) & 1u)
                 ip_pair<AK>(k, vr[p], vi[p], vr[q], vi[q]);
         }
     }
@@ -235,26 +233,25 @@ __device__ __forceinline__ void pair_j(double *vr, double *vi, const double *k, 
 
 template <int R, int RB, int AK, bool PRED>
 __device__ __forceinline__ void pair_sel(uint32_t sel, double *vr, double *vi, const double *k,
-                                         uint32_t mro, uint32_t mrz, bool lane_ok)
+                                         uint32_t pm, bool lane_ok)
 {
-    if (sel & 1u) pair_j<R, AK, 0, PRED>(vr, vi, k, mro, mrz, lane_ok);
-    if constexpr (RB > 1) if (sel & 2u) pair_j<R, AK, 1, PRED>(vr, vi, k, mro, mrz, lane_ok);
-    if constexpr (RB > 2) if (sel & 4u) pair_j<R, AK, 2, PRED>(vr, vi, k, mro, mrz, lane_ok);
-    if constexpr (RB > 3) if (sel & 8u) pair_j<R, AK, 3, PRED>(vr, vi, k, mro, mrz, lane_ok);
+    if (sel & 1u) pair_j<R, AK, 0, PRED>(vr, vi, k, pm, lane_ok);
+    if constexpr (RB > 1) if (sel & 2u) pair_j<R, AK, 1, PRED>(vr, vi, k, pm, lane_ok);
+    if constexpr (RB > 2) if (sel & 4u) pair_j<R, AK, 2, PRED>(vr, vi, k, pm, lane_ok);
+    if constexpr (RB > 3) if (sel & 8u) pair_j<R, AK, 3, PRED>(vr, vi, k, pm, lane_ok);
 }
 
 template <int R, int RB, bool PRED>
 __device__ __forceinline__ void pair_kinds(uint32_t g, uint32_t sel, double *vr, double *vi,
-                                           const double *k, uint32_t mro, uint32_t mrz,
+                                           const double *k, uint32_t pm,
                                            bool lane_ok)
 {
-    if (g & TG_H) pair_sel<R, RB, AK_H, PRED>(sel, vr, vi, k, mro, mrz, lane_ok);
-    if (g & TG_X) pair_sel<R, RB, AK_X, PRED>(sel, vr, vi, k, mro, mrz, lane_ok);
-    if (g & TG_Y) pair_sel<R, RB, AK_Y, PRED>(sel, vr, vi, k, mro, mrz, lane_ok);
-    if (g & TG_RX) pair_sel<R, RB, AK_RX, PRED>(sel, vr, vi, k, mro, mrz, lane_ok);
-    if (g & TG_RY) pair_sel<R, RB, AK_RY, PRED>(sel, vr, vi, k, mro, mrz, lane_ok);
-    if (g & TG_U) pair_sel<R, RB, AK_U, PRED>(sel, vr, vi, k, mro, mrz, lane_ok);
-    if (g & TG_MAT) pair_sel<R, RB, AK_MAT, PRED>(sel, vr, vi, k, mro, mrz, lane_ok);
+    if (g & TG_H) pair_sel<R, RB, AK_H, PRED>(sel, vr, vi, k, pm, lane_ok);
+    if (g & TG_X) pair_sel<R, RB, AK_X, PRED>(sel, vr, vi, k, pm, lane_ok);
+    if (g & TG_Y) pair_sel<R, RB, AK_Y, PRED>(sel, vr, vi, k, pm, lane_ok);
+    if (g & TG_RX) pair_sel<R, RB, AK_RX, PRED>(sel, vr, vi, k, pm, lane_ok);
+    if (g & TG_RY) pair_sel<R, RB, AK_RY, PRED>(sel, vr, vi, k, pm, lane_ok);
+    if (g & TG_U) pair_sel<R, RB, AK_U, PRED>(sel, vr, vi, k, pm, lane_ok);
 }
 
 // Diagonal update of one amplitude whose target bit is known at compile
@@ -275,8 +272,7 @@ __device__ __forceinline__ void diag_known(const double *k, bool bit, double &x,
 
 // Diagonal target on register bit J.
 template <int R, int AK, int J, bool PRED>
-__device__ __forceinline__ void diag_j(double *vr, double *vi, const double *k, uint32_t mro,
-                                       uint32_t mrz, bool lane_ok)
+__device__ __forceinline__ void diag_j(double *vr, double *vi, const double *k, uint32_t pm, bool lane_ok)
 {
 #pragma unroll
     for (int p = 0; p < R; p++) {
@@ -286,7 +282,7 @@ __device__ __forceinline__ void diag_j(double *vr, double *vi, const double *k, 
         if constexpr (!PRED) {
             diag_known<AK>(k, bit, vr[p], vi[p]);
         } else {
-            if (((uint32_t)p & mro) == mro && !((uint32_t)p & mrz))
+            if ((pm >> p) & 1u)
                 diag_known<AK>(k, bit, vr[p], vi[p]);
         }
     }
@@ -294,19 +290,19 @@ __device__ __forceinline__ void diag_j(double *vr, double *vi, const double *k, 
 
 template <int R, int RB, int AK, bool PRED>
 __device__ __forceinline__ void diag_sel(uint32_t sel, double *vr, double *vi, const double *k,
-                                         uint32_t mro, uint32_t mrz, bool lane_ok)
+                                         uint32_t pm, bool lane_ok)
 {
-    if (sel & 1u) diag_j<R, AK, 0, PRED>(vr, vi, k, mro, mrz, lane_ok);
-    if constexpr (RB > 1) if (sel & 2u) diag_j<R, AK, 1, PRED>(vr, vi, k, mro, mrz, lane_ok);
-    if constexpr (RB > 2) if (sel & 4u) diag_j<R, AK, 2, PRED>(vr, vi, k, mro, mrz, lane_ok);
-    if constexpr (RB > 3) if (sel & 8u) diag_j<R, AK, 3, PRED>(vr, vi, k, mro, mrz, lane_ok);
+    if (sel & 1u) diag_j<R, AK, 0, PRED>(vr, vi, k, pm, lane_ok);
+    if constexpr (RB > 1) if (sel & 2u) diag_j<R, AK, 1, PRED>(vr, vi, k, pm, lane_ok);
+    if constexpr (RB > 2) if (sel & 4u) diag_j<R, AK, 2, PRED>(vr, vi, k, pm, lane_ok);
+    if constexpr (RB > 3) if (sel & 8u) diag_j<R, AK, 3, PRED>(vr, vi, k, pm, lane_ok);
 }
 
 // Diagonal target on a lane bit or outside the tile: `bit` per lane (or per
 // tile); the RZ multipliers are selected once per op.
 template <int R, int AK, bool PRED>
 __device__ __forceinline__ void diag_lane(double *vr, double *vi, const double *k, bool bit,
-                                          uint32_t mro, uint32_t mrz, bool lane_ok)
+                                          uint32_t pm, bool lane_ok)
 {
     if constexpr (AK == AK_RZ) {
         const double mr = bit ? k[2] : k[0];
@@ -316,7 +312,7 @@ __device__ __forceinline__ void diag_lane(double *vr, double *vi, const double *
             if constexpr (!PRED) {
                 ip_rz<true>(mr, mi, vr[p], vi[p]);
             } else {
-                if (((uint32_t)p & mro) == mro && !((uint32_t)p & mrz))
+                if ((pm >> p) & 1u)
                     ip_rz<true>(mr, mi, vr[p], vi[p]);
             }
         }
@@ -328,7 +324,7 @@ __device__ __forceinline__ void diag_lane(double *vr, double *vi, const double *
             if constexpr (!PRED) {
                 ip_scale(sg, vr[p], vi[p]);
             } else {
-                if (((uint32_t)p & mro) == mro && !((uint32_t)p & mrz))
+                if ((pm >> p) & 1u)
                     ip_scale(sg, vr[p], vi[p]);
             }
         }
@@ -338,7 +334,7 @@ __device__ __forceinline__ void diag_lane(double *vr, double *vi, const double *
             if constexpr (!PRED) {
                 diag_known<AK>(k, true, vr[p], vi[p]);
             } else {
-                if (((uint32_t)p & mro) == mro && !((uint32_t)p & mrz))
+                if ((pm >> p) & 1u)
                     diag_known<AK>(k, true, vr[p], vi[p]);
             }
         }
@@ -347,22 +343,21 @@ __device__ __forceinline__ void diag_lane(double *vr, double *vi, const double *
 
 template <int R, int RB, bool PRED>
 __device__ __forceinline__ void diag_reg_kinds(uint32_t g, uint32_t sel, double *vr, double *vi,
-                                               const double *k, uint32_t mro, uint32_t mrz,
+                                               const double *k, uint32_t pm,
                                                bool lane_ok)
 {
-    if (g & TG_NEG) diag_sel<R, RB, AK_NEG, PRED>(sel, vr, vi, k, mro, mrz, lane_ok);
-    if (g & TG_PHASE) diag_sel<R, RB, AK_PHASE, PRED>(sel, vr, vi, k, mro, mrz, lane_ok);
-    if (g & TG_RZ) diag_sel<R, RB, AK_RZ, PRED>(sel, vr, vi, k, mro, mrz, lane_ok);
+    if (g & TG_NEG) diag_sel<R, RB, AK_NEG, PRED>(sel, vr, vi, k, pm, lane_ok);
+    if (g & TG_PHASE) diag_sel<R, RB, AK_PHASE, PRED>(sel, vr, vi, k, pm, lane_ok);
+    if (g & TG_RZ) diag_sel<R, RB, AK_RZ, PRED>(sel, vr, vi, k, pm, lane_ok);
 }
 
 template <int R, bool PRED>
 __device__ __forceinline__ void diag_lane_kinds(uint32_t g, double *vr, double *vi,
-                                                const double *k, bool bit, uint32_t mro,
-                                                uint32_t mrz, bool lane_ok)
+                                                const double *k, bool bit, uint32_t pm, bool lane_ok)
 {
-    if (g & TG_NEG) diag_lane<R, AK_NEG, PRED>(vr, vi, k, bit, mro, mrz, lane_ok);
-    if (g & TG_PHASE) diag_lane<R, AK_PHASE, PRED>(vr, vi, k, bit, mro, mrz, lane_ok);
-    if (g & TG_RZ) diag_lane<R, AK_RZ, PRED>(vr, vi, k, bit, mro, mrz, lane_ok);
+    if (g & TG_NEG) diag_lane<R, AK_NEG, PRED>(vr, vi, k, bit, pm, lane_ok);
+    if (g & TG_PHASE) diag_lane<R, AK_PHASE, PRED>(vr, vi, k, bit, pm, lane_ok);
+    if (g & TG_RZ) diag_lane<R, AK_RZ, PRED>(vr, vi, k, bit, pm, lane_ok);
 }
 
 // Register part of the local index of register p under a layout.
@@ -376,8 +371,12 @@ __device__ __forceinline__ uint32_t reg_part(int p, const int8_t *regbits)
     return r;
 }
 
+// Register budget: asking for 6 waves per SIMD makes the compiler fit the
+// kernel in ~102 VGPRs without spills (4 resident waves per SIMD instead of 3
+// at its unconstrained 138); the extra wave lets one workgroup's loads and
+// stores overlap the others' gate arithmetic.
 template <typename T, int K, int LOGW>
-__global__ void __launch_bounds__(1 << LOGW)
+__global__ void __launch_bounds__(1 << LOGW) __attribute__((amdgpu_waves_per_eu(6, 6)))
 k_tile(T *__restrict__ re, T *__restrict__ im, const TileDesc d, const TileOp *__restrict__ ops)
 {
     constexpr int RB = K - LOGW;
@@ -441,25 +440,25 @@ k_tile(T *__restrict__ re, T *__restrict__ im, const TileDesc d, const TileOp *_
         }
         // (a control outside the tile that is not met skips the op)
         if ((g & TG_GATE) && (base & cur.eo) == cur.eo && !(base & cur.ez)) {
-            const uint32_t sel = cur.sel, mro = cur.mro, mrz = cur.mrz;
+            const uint32_t sel = cur.sel, pm = cur.pm;
             // controls on lane bits: a per-lane predicate inside the predicated paths
             const bool lane_ok = (lt & cur.mlo) == cur.mlo && !(lt & cur.mlz);
             const double *k = cur.c;
-            if (g & TG_PAIR) pair_kinds<R, RB, false>(g, sel, vr, vi, k, 0, 0, true);
+            if (g & TG_PAIR) pair_kinds<R, RB, false>(g, sel, vr, vi, k, 0, true);
             // predicated ops: the lane part of the controls is one divergent
             // branch around the op, the register part a uniform test per register
             if (g & TG_PAIRP)
-                if (lane_ok) pair_kinds<R, RB, true>(g, sel, vr, vi, k, mro, mrz, true);
-            if (g & TG_DREG) diag_reg_kinds<R, RB, false>(g, sel, vr, vi, k, 0, 0, true);
+                if (lane_ok) pair_kinds<R, RB, true>(g, sel, vr, vi, k, pm, true);
+            if (g & TG_DREG) diag_reg_kinds<R, RB, false>(g, sel, vr, vi, k, 0, true);
             if (g & TG_DREGP)
-                if (lane_ok) diag_reg_kinds<R, RB, true>(g, sel, vr, vi, k, mro, mrz, true);
+                if (lane_ok) diag_reg_kinds<R, RB, true>(g, sel, vr, vi, k, pm, true);
             if (g & (TG_DLANE | TG_DLANEP)) {
                 // target on a lane bit (dsel 1) or outside the tile (dsel 2)
                 const bool bit = cur.dsel == 2 ? (base & cur.ebit) != 0
                                                : ((lt >> cur.dpos) & 1u) != 0;
-                if (g & TG_DLANE) diag_lane_kinds<R, false>(g, vr, vi, k, bit, 0, 0, true);
+                if (g & TG_DLANE) diag_lane_kinds<R, false>(g, vr, vi, k, bit, 0, true);
                 if (g & TG_DLANEP)
-                    if (lane_ok) diag_lane_kinds<R, true>(g, vr, vi, k, bit, mro, mrz, true);
+                    if (lane_ok) diag_lane_kinds<R, true>(g, vr, vi, k, bit, pm, true);
             }
         }
     }
