@@ -36,8 +36,10 @@ COMMON = [
 # near-identical case bodies into one block with a computed register index,
 # which forces the amplitude arrays into scratch memory (272 B/lane, ~25x
 # slower).  Keeping the cases apart keeps them in VGPRs.
+# The tile kernel also asks for a 6-waves-per-SIMD register budget (see its
+# source); the compiler reaches 4 without spills and warns that 6 was not met.
 EXTRA = {
-    "spz_tile.hip": ["-mllvm", "-simplifycfg-sink-common=false"],
+    "spz_tile.hip": ["-mllvm", "-simplifycfg-sink-common=false", "-Wno-pass-failed"],
 }
 
 

@@ -23,6 +23,7 @@ namespace spz {
 
 int adjoint_grad(spz_state_s *s, const spz_op *ops, uint64_t n_ops, double hx, int ckpt_every,
                  double *grad, double *energy, double *drift);
+int adjoint_reserve(int n, int prec, uint64_t n_ops);
 
 namespace {
 
@@ -225,6 +226,18 @@ int lower_bop(const spz_op &o, BOp *b)
 
 int g_wg_max_qubits = -1;  // autotuned switch point (-1: not tuned yet)
 
+// A second stream (default priority) for the persistent small-circuit kernel,
+// so that it runs beside the single-circuit path of the larger circuits.
+hipStream_t side_stream()
+{
+    static hipStream_t st = nullptr;
+    if (!st && hipStreamCreateWithFlags(&st, hipStreamNonBlocking) != hipSuccess) {
+        (void)hipGetLastError();
+        st = nullptr;
+    }
+    return st;
+}
+
 }  // namespace
 
 }  // namespace spz
@@ -259,8 +272,51 @@ int spz_batch_adjoint(const spz_batch_circuit *circuits, uint64_t count, const s
     (void)total_ops;
     auto t0 = std::chrono::steady_clock::now();
     std::vector<double> lat(count, 0.0);
-    // ---- small circuits: one persistent launch ---------------------------------
+    // ---- larger circuits: pre-size everything they need, so that no
+    // allocation or free (which would synchronise the device) happens while
+    // the small-circuit kernel runs beside them
+    std::stable_sort(large.begin(), large.end(),
+                     [&](uint64_t x, uint64_t y) { return circuits[x].n < circuits[y].n; });
+    std::vector<spz_state> st_by_n(64, nullptr);
+    auto release = [&]() {
+        for (spz_state &x : st_by_n)
+            if (x) spz_state_destroy(x);
+    };
+    if (!large.empty()) {
+        int nmax = 0;
+        uint64_t opsmax = 0;
+        for (uint64_t i : large) {
+            nmax = std::max(nmax, circuits[i].n);
+            opsmax = std::max<uint64_t>(opsmax, (uint64_t)circuits[i].n_ops);
+        }
+        int rc = adjoint_reserve(nmax, SPZ_F64, opsmax);
+        for (uint64_t i : large)
+            if (rc == SPZ_OK && !st_by_n[circuits[i].n])
+                rc = spz_state_create(circuits[i].n, SPZ_F64, &st_by_n[circuits[i].n]);
+        if (rc != SPZ_OK) {
+            release();
+            return rc;
+        }
+    }
+    // ---- small circuits: one persistent launch on the side stream ----------------
+    void *d_circ = nullptr, *d_ops = nullptr, *d_scr = nullptr, *d_cnt = nullptr,
+         *d_e = nullptr, *d_g = nullptr, *d_t = nullptr;
+    auto cleanup = [&]() {
+        for (void *p : {d_circ, d_ops, d_scr, d_cnt, d_e, d_g, d_t})
+            if (p) (void)hipFree(p);
+    };
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    uint64_t gmax = 0;
+    hipStream_t st = side_stream() ? side_stream() : rt().stream;
     if (!small.empty()) {
+        // largest first: the long circuits start at once and the short ones
+        // fill the workgroups that finish early
+        std::stable_sort(small.begin(), small.end(), [&](uint64_t x, uint64_t y) {
+            const double cx = (double)circuits[x].n_ops * (double)(1ull << circuits[x].n);
+            const double cy = (double)circuits[y].n_ops * (double)(1ull << circuits[y].n);
+            return cx > cy;
+        });
         std::vector<BCirc> bc(small.size());
         std::vector<BOp> bo;
         int nmax = 1;
@@ -273,7 +329,11 @@ int spz_batch_adjoint(const spz_batch_circuit *circuits, uint64_t count, const s
             nmax = std::max(nmax, c.n);
             for (int o = 0; o < c.n_ops; o++) {
                 BOp b;
-                SPZ_TRY(lower_bop(ops[c.op_offset + o], &b));
+                int rc = lower_bop(ops[c.op_offset + o], &b);
+                if (rc != SPZ_OK) {
+                    release();
+                    return rc;
+                }
                 bo.push_back(b);
             }
         }
@@ -284,15 +344,7 @@ int spz_batch_adjoint(const spz_batch_circuit *circuits, uint64_t count, const s
         grid = std::min<uint64_t>(grid, small.size());
         // scratch: 4 arrays of 2^nmax doubles per workgroup
         const uint64_t scratch_bytes = grid * 4 * slot * sizeof(double);
-        void *d_circ = nullptr, *d_ops = nullptr, *d_scr = nullptr, *d_cnt = nullptr,
-             *d_e = nullptr, *d_g = nullptr, *d_t = nullptr;
-        uint64_t gmax = 0;
         for (uint64_t i : small) gmax = std::max<uint64_t>(gmax, circuits[i].grad_offset + circuits[i].n_ops);
-        int rc = SPZ_OK;
-        auto cleanup = [&]() {
-            for (void *p : {d_circ, d_ops, d_scr, d_cnt, d_e, d_g, d_t})
-                if (p) (void)hipFree(p);
-        };
         if (hipMalloc(&d_circ, bc.size() * sizeof(BCirc)) != hipSuccess ||
             hipMalloc(&d_ops, std::max<size_t>(bo.size(), 1) * sizeof(BOp)) != hipSuccess ||
             hipMalloc(&d_scr, scratch_bytes) != hipSuccess ||
@@ -300,60 +352,85 @@ int spz_batch_adjoint(const spz_batch_circuit *circuits, uint64_t count, const s
             hipMalloc(&d_e, small.size() * sizeof(double)) != hipSuccess ||
             hipMalloc(&d_g, std::max<uint64_t>(gmax, 1) * sizeof(double)) != hipSuccess ||
             hipMalloc(&d_t, small.size() * 2 * sizeof(unsigned long long)) != hipSuccess) {
+            (void)hipGetLastError();
             cleanup();
+            release();
             return fail(SPZ_ERR_OOM, "batched adjoint scratch allocation failed");
-        }
-        hipStream_t st = rt().stream;
-        if (hipMemcpyAsync(d_circ, bc.data(), bc.size() * sizeof(BCirc), hipMemcpyHostToDevice, st) != hipSuccess ||
-            (bo.size() && hipMemcpyAsync(d_ops, bo.data(), bo.size() * sizeof(BOp), hipMemcpyHostToDevice, st) != hipSuccess) ||
-            hipStreamSynchronize(st) != hipSuccess) {
-            cleanup();
-            return fail(SPZ_ERR_HIP, "batched adjoint upload failed");
         }
         // the gradient slots of parameter-free ops stay zero
         std::vector<double> zeros(std::max<uint64_t>(gmax, 1), 0.0);
-        (void)hipMemcpy(d_g, zeros.data(), zeros.size() * sizeof(double), hipMemcpyHostToDevice);
-        {
-            LaunchTimer tm(SPZ_K_ADJOINT, 0.0);
-            if (use_graph) {
-                hipGraph_t graph = nullptr;
-                hipGraphExec_t exec = nullptr;
-                bool ok = hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal) == hipSuccess;
-                if (ok) {
-                    hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(1), 0, st, (unsigned int *)d_cnt);
-                    hipLaunchKernelGGL(k_batch_adjoint, dim3((unsigned)grid), dim3(kW), 0, st,
-                                       (const BCirc *)d_circ, (uint64_t)small.size(),
-                                       (const BOp *)d_ops, hx, (double *)d_scr, slot,
-                                       (unsigned int *)d_cnt, (double *)d_e, (double *)d_g,
-                                       (unsigned long long *)d_t);
-                    ok = hipStreamEndCapture(st, &graph) == hipSuccess;
-                }
-                if (ok) ok = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0) == hipSuccess;
-                if (ok) ok = hipGraphLaunch(exec, st) == hipSuccess;
-                if (exec) (void)hipGraphExecDestroy(exec);
-                if (graph) (void)hipGraphDestroy(graph);
-                if (!ok) rc = fail(SPZ_ERR_HIP, "batched adjoint graph launch failed");
-            } else {
+        if (hipMemcpy(d_circ, bc.data(), bc.size() * sizeof(BCirc), hipMemcpyHostToDevice) != hipSuccess ||
+            (bo.size() && hipMemcpy(d_ops, bo.data(), bo.size() * sizeof(BOp), hipMemcpyHostToDevice) != hipSuccess) ||
+            hipMemcpy(d_g, zeros.data(), zeros.size() * sizeof(double), hipMemcpyHostToDevice) != hipSuccess) {
+            cleanup();
+            release();
+            return fail(SPZ_ERR_HIP, "batched adjoint upload failed");
+        }
+        bool ok = true;
+        if (use_graph) {
+            ok = hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal) == hipSuccess;
+            if (ok) {
                 hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(1), 0, st, (unsigned int *)d_cnt);
                 hipLaunchKernelGGL(k_batch_adjoint, dim3((unsigned)grid), dim3(kW), 0, st,
                                    (const BCirc *)d_circ, (uint64_t)small.size(),
                                    (const BOp *)d_ops, hx, (double *)d_scr, slot,
                                    (unsigned int *)d_cnt, (double *)d_e, (double *)d_g,
                                    (unsigned long long *)d_t);
-                if (hipGetLastError() != hipSuccess) rc = fail(SPZ_ERR_HIP, "batched adjoint launch failed");
+                ok = hipStreamEndCapture(st, &graph) == hipSuccess;
             }
+            if (ok) ok = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0) == hipSuccess;
+            if (ok) ok = hipGraphLaunch(exec, st) == hipSuccess;
+        } else {
+            hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(1), 0, st, (unsigned int *)d_cnt);
+            hipLaunchKernelGGL(k_batch_adjoint, dim3((unsigned)grid), dim3(kW), 0, st,
+                               (const BCirc *)d_circ, (uint64_t)small.size(), (const BOp *)d_ops,
+                               hx, (double *)d_scr, slot, (unsigned int *)d_cnt, (double *)d_e,
+                               (double *)d_g, (unsigned long long *)d_t);
+            ok = hipGetLastError() == hipSuccess;
         }
-        if (rc == SPZ_OK && hipStreamSynchronize(st) != hipSuccess)
-            rc = fail(SPZ_ERR_HIP, "batched adjoint kernel failed");
-        if (rc == SPZ_OK) {
+        if (!ok) {
+            (void)hipGetLastError();
+            (void)hipStreamSynchronize(st);
+            if (exec) (void)hipGraphExecDestroy(exec);
+            if (graph) (void)hipGraphDestroy(graph);
+            cleanup();
+            release();
+            return fail(SPZ_ERR_HIP, "batched adjoint launch failed");
+        }
+    }
+    // ---- larger circuits: the single-circuit path on the library stream -----------
+    int rc_large = SPZ_OK;
+    for (uint64_t i : large) {
+        const spz_batch_circuit &c = circuits[i];
+        spz_state s = st_by_n[c.n];
+        int rc = spz_state_set_basis(s, 0);
+        auto a = std::chrono::steady_clock::now();
+        if (rc == SPZ_OK)
+            rc = adjoint_grad(s, ops + c.op_offset, (uint64_t)c.n_ops, hx, 0, grads + c.grad_offset,
+                              &energies[i], nullptr);
+        auto b = std::chrono::steady_clock::now();
+        if (rc != SPZ_OK) {
+            rc_large = rc;
+            break;
+        }
+        lat[i] = std::chrono::duration<double, std::milli>(b - a).count();
+    }
+    // ---- collect the small circuits ---------------------------------------------
+    int rc_small = SPZ_OK;
+    if (!small.empty()) {
+        if (hipStreamSynchronize(st) != hipSuccess) rc_small = fail(SPZ_ERR_HIP, "batched adjoint kernel failed");
+        if (exec) (void)hipGraphExecDestroy(exec);
+        if (graph) (void)hipGraphDestroy(graph);
+        if (rc_small == SPZ_OK) {
             std::vector<double> e(small.size()), g(std::max<uint64_t>(gmax, 1));
             std::vector<unsigned long long> t(2 * small.size());
-            (void)hipMemcpy(e.data(), d_e, e.size() * sizeof(double), hipMemcpyDeviceToHost);
-            (void)hipMemcpy(g.data(), d_g, g.size() * sizeof(double), hipMemcpyDeviceToHost);
-            (void)hipMemcpy(t.data(), d_t, t.size() * sizeof(unsigned long long), hipMemcpyDeviceToHost);
+            if (hipMemcpy(e.data(), d_e, e.size() * sizeof(double), hipMemcpyDeviceToHost) != hipSuccess ||
+                hipMemcpy(g.data(), d_g, g.size() * sizeof(double), hipMemcpyDeviceToHost) != hipSuccess ||
+                hipMemcpy(t.data(), d_t, t.size() * sizeof(unsigned long long), hipMemcpyDeviceToHost) != hipSuccess)
+                rc_small = fail(SPZ_ERR_HIP, "batched adjoint download failed");
             int clk_khz = 100000;  // wall_clock64 runs at 100 MHz on gfx9
             (void)hipDeviceGetAttribute(&clk_khz, hipDeviceAttributeWallClockRate, rt().device);
-            for (size_t k = 0; k < small.size(); k++) {
+            for (size_t k = 0; rc_small == SPZ_OK && k < small.size(); k++) {
                 const uint64_t i = small[k];
                 energies[i] = e[k];
                 const spz_batch_circuit &c = circuits[i];
@@ -361,40 +438,11 @@ int spz_batch_adjoint(const spz_batch_circuit *circuits, uint64_t count, const s
                 lat[i] = (double)(t[2 * k + 1] - t[2 * k]) / (double)clk_khz;  // ms
             }
         }
-        cleanup();
-        if (rc != SPZ_OK) return rc;
     }
-    // ---- large circuits: the single-circuit path, one after another -------------
-    // (one state per qubit count, reset to |0...0> before each circuit; sorted
-    // by qubit count so the adjoint's working states are reshaped rarely)
-    std::stable_sort(large.begin(), large.end(),
-                     [&](uint64_t x, uint64_t y) { return circuits[x].n < circuits[y].n; });
-    std::vector<spz_state> st_by_n(64, nullptr);
-    auto release = [&]() {
-        for (spz_state &x : st_by_n)
-            if (x) spz_state_destroy(x);
-    };
-    for (uint64_t i : large) {
-        const spz_batch_circuit &c = circuits[i];
-        spz_state &s = st_by_n[c.n];
-        int rc = SPZ_OK;
-        if (!s) rc = spz_state_create(c.n, SPZ_F64, &s);
-        else rc = spz_state_set_basis(s, 0);
-        if (rc != SPZ_OK) {
-            release();
-            return rc;
-        }
-        auto a = std::chrono::steady_clock::now();
-        rc = adjoint_grad(s, ops + c.op_offset, (uint64_t)c.n_ops, hx, 0, grads + c.grad_offset,
-                          &energies[i], nullptr);
-        auto b = std::chrono::steady_clock::now();
-        if (rc != SPZ_OK) {
-            release();
-            return rc;
-        }
-        lat[i] = std::chrono::duration<double, std::milli>(b - a).count();
-    }
+    cleanup();
     release();
+    if (rc_large != SPZ_OK) return rc_large;
+    if (rc_small != SPZ_OK) return rc_small;
     auto t1 = std::chrono::steady_clock::now();
     if (stats) {
         std::vector<double> l = lat;
