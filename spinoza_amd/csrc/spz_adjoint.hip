@@ -22,7 +22,9 @@
 #include <mutex>
 #include <vector>
 
+#include "spz_inplace.h"
 #include "spz_internal.h"
+#include "spz_tile.h"
 
 namespace spz {
 
@@ -64,7 +66,8 @@ constexpr int kGMaxQ = 3;
 constexpr int kGPerThread = 4;  // elements per thread
 
 struct GroupOp {
-    int32_t ak, gk, tq, pad;  // arithmetic, gradient kind, index of the target in Q
+    int32_t ak, gk, tq;       // arithmetic, gradient kind, index of the target in Q
+    uint32_t kb;              // one-hot kind bit (TileGroup TG_H ... TG_RZ) of ak
     uint64_t cm, ones;        // control mask and the control bits that must be 1
     double c[8];              // inverse-gate coefficients
 };
@@ -88,9 +91,24 @@ __device__ __forceinline__ double grad_term(int gk, double la, double lb, double
     return -2.0 * im_cj(lc, ld, c, d);  // P: -2 Im <l|Pi_1|p>
 }
 
+// The inverse gate on one pair, in place (the kind is a compile-time constant).
+template <int AK>
+__device__ __forceinline__ void inv_pair(const double *k, double &a, double &b, double &c, double &d)
+{
+    if constexpr (AK == AK_H) ip_h(k[0], a, b, c, d);
+    else if constexpr (AK == AK_X) { ip_swap(a, c); ip_swap(b, d); }
+    else if constexpr (AK == AK_Y) ip_y(a, b, c, d);
+    else if constexpr (AK == AK_RX) ip_rx(k[0], k[1], a, b, c, d);
+    else if constexpr (AK == AK_RY) ip_ry(k[0], k[1], a, b, c, d);
+    else if constexpr (AK == AK_U) ip_u(k, a, b, c, d);
+    else if constexpr (AK == AK_NEG) { c = -c; d = -d; }
+    else if constexpr (AK == AK_PHASE) ip_phase(k[0], k[1], c, d);
+    else { ip_rz<false>(k[0], k[1], a, b); ip_rz<false>(k[2], k[3], c, d); }  // AK_RZ
+}
+
 // Apply op (target = Q[TQ]) to the pairs of one element, adding the gradient
 // terms of the pairs whose controls are met to (s, e).
-template <int M, int TQ>
+template <int M, int TQ, int AK>
 __device__ __forceinline__ void group_apply(const GroupOp &op, uint64_t base, const uint64_t *off,
                                             double *ar, double *ai, double *lr, double *li,
                                             double &s, double &e)
@@ -107,9 +125,28 @@ __device__ __forceinline__ void group_apply(const GroupOp &op, uint64_t base, co
             s = ns;
             e = e + er;
         }
-        pair_update(op.ak, op.c, ar[x], ai[x], ar[y], ai[y]);
-        pair_update(op.ak, op.c, lr[x], li[x], lr[y], li[y]);
+        inv_pair<AK>(op.c, ar[x], ai[x], ar[y], ai[y]);
+        inv_pair<AK>(op.c, lr[x], li[x], lr[y], li[y]);
     }
+}
+
+// Dispatch on the op's kind through independent tests of one-hot bits (a
+// multi-way branch would merge the register arrays of every path and copy them).
+template <int M, int TQ>
+__device__ __forceinline__ void group_kinds(const GroupOp &op, uint64_t base, const uint64_t *off,
+                                            double *ar, double *ai, double *lr, double *li,
+                                            double &s, double &e)
+{
+    const uint32_t kb = op.kb;
+    if (kb & TG_H) group_apply<M, TQ, AK_H>(op, base, off, ar, ai, lr, li, s, e);
+    if (kb & TG_X) group_apply<M, TQ, AK_X>(op, base, off, ar, ai, lr, li, s, e);
+    if (kb & TG_Y) group_apply<M, TQ, AK_Y>(op, base, off, ar, ai, lr, li, s, e);
+    if (kb & TG_RX) group_apply<M, TQ, AK_RX>(op, base, off, ar, ai, lr, li, s, e);
+    if (kb & TG_RY) group_apply<M, TQ, AK_RY>(op, base, off, ar, ai, lr, li, s, e);
+    if (kb & TG_U) group_apply<M, TQ, AK_U>(op, base, off, ar, ai, lr, li, s, e);
+    if (kb & TG_NEG) group_apply<M, TQ, AK_NEG>(op, base, off, ar, ai, lr, li, s, e);
+    if (kb & TG_PHASE) group_apply<M, TQ, AK_PHASE>(op, base, off, ar, ai, lr, li, s, e);
+    if (kb & TG_RZ) group_apply<M, TQ, AK_RZ>(op, base, off, ar, ai, lr, li, s, e);
 }
 
 // partials[(j * gridDim.x + block) * 2 + {0, 1}]: compensated block sum of
@@ -152,9 +189,10 @@ k_adj_group(T *__restrict__ pr, T *__restrict__ pi, T *__restrict__ qr, T *__res
             for (int j = 0; j < nop; j++) {
                 const GroupOp &op = gd.op[j];
                 double s = S[j][tid], e = C[j][tid];
-                if (op.tq == 0) group_apply<M, 0>(op, base, off, ar, ai, lr, li, s, e);
-                if constexpr (M > 1) if (op.tq == 1) group_apply<M, 1>(op, base, off, ar, ai, lr, li, s, e);
-                if constexpr (M > 2) if (op.tq == 2) group_apply<M, 2>(op, base, off, ar, ai, lr, li, s, e);
+                const uint32_t tqb = 1u << op.tq;  // one-hot, tested independently
+                if (tqb & 1u) group_kinds<M, 0>(op, base, off, ar, ai, lr, li, s, e);
+                if constexpr (M > 1) if (tqb & 2u) group_kinds<M, 1>(op, base, off, ar, ai, lr, li, s, e);
+                if constexpr (M > 2) if (tqb & 4u) group_kinds<M, 2>(op, base, off, ar, ai, lr, li, s, e);
                 S[j][tid] = s;
                 C[j][tid] = e;
             }
@@ -278,6 +316,22 @@ unsigned maxdiff_grid(uint64_t N)
     return (unsigned)std::max<uint64_t>(1, std::min<uint64_t>(cap, (N + kT - 1) / kT));
 }
 
+uint32_t kind_bit_of(int ak)
+{
+    switch (ak) {
+    case AK_H: return TG_H;
+    case AK_X: return TG_X;
+    case AK_Y: return TG_Y;
+    case AK_RX: return TG_RX;
+    case AK_RY: return TG_RY;
+    case AK_U: return TG_U;
+    case AK_NEG: return TG_NEG;
+    case AK_PHASE: return TG_PHASE;
+    case AK_RZ: return TG_RZ;
+    default: return 0;
+    }
+}
+
 bool has_param(int kind)
 {
     return kind == SPZ_RX || kind == SPZ_RY || kind == SPZ_RZ || kind == SPZ_P;
@@ -345,31 +399,53 @@ int make_like(const spz_state_s *s, OwnedState &o)
 }
 
 // Scratch kept between calls (hipFree synchronises the device, and small
-// circuits would otherwise spend most of their time allocating): the two
-// working states of the last shape used and grow-only device buffers.  One
-// lock serialises the adjoint calls that share it.
+// circuits would otherwise spend most of their time allocating): four
+// grow-only arrays that back the two working states (psi copy and lambda,
+// viewed with the current shape) and grow-only device buffers.  One lock
+// serialises the adjoint calls that share it.
 struct AdjScratch {
     std::mutex mu;
-    OwnedState work, lam;
+    void *arr[4] = {nullptr, nullptr, nullptr, nullptr};  // work re/im, lambda re/im
+    size_t arr_cap = 0;
+    spz_state_s work, lam;  // views into arr
     void *grad = nullptr, *part = nullptr, *dmax = nullptr;
     size_t grad_cap = 0, part_cap = 0;
 
+    int reserve_states(int n, int prec)
+    {
+        const size_t bytes = ((size_t)1 << n) * (prec == SPZ_F64 ? 8 : 4);
+        if (bytes > arr_cap) {
+            for (void *&a : arr)
+                if (a) {
+                    SPZ_HIP(hipFree(a));
+                    a = nullptr;
+                }
+            arr_cap = 0;
+            for (void *&a : arr)
+                if (hipMalloc(&a, bytes) != hipSuccess) {
+                    (void)hipGetLastError();
+                    for (void *&b : arr)
+                        if (b) {
+                            (void)hipFree(b);
+                            b = nullptr;
+                        }
+                    return fail(SPZ_ERR_OOM, "adjoint scratch allocation failed");
+                }
+            arr_cap = bytes;
+        }
+        return SPZ_OK;
+    }
+
     int states_like(const spz_state_s *s)
     {
-        if (work.s && (work.s->n != s->n || work.s->prec != s->prec)) {
-            spz_state_destroy(work.s);
-            spz_state_destroy(lam.s);
-            work.s = lam.s = nullptr;
-        }
-        if (!work.s) {
-            SPZ_TRY(make_like(s, work));
-            int rc = make_like(s, lam);
-            if (rc != SPZ_OK) {
-                spz_state_destroy(work.s);
-                work.s = nullptr;
-                return rc;
-            }
-        }
+        SPZ_TRY(reserve_states(s->n, s->prec));
+        work.n = lam.n = s->n;
+        work.prec = lam.prec = s->prec;
+        work.N = lam.N = s->N;
+        work.re = arr[0];
+        work.im = arr[1];
+        lam.re = arr[2];
+        lam.im = arr[3];
         return SPZ_OK;
     }
 
@@ -403,6 +479,14 @@ struct AdjScratch {
     }
 };
 
+size_t grad_bytes(uint64_t n_ops) { return std::max<uint64_t>(n_ops, 1) * sizeof(double); }
+
+size_t part_bytes(uint64_t N)
+{
+    const uint64_t per_block = (uint64_t)kT * kGPerThread;
+    return ((N / 2 + per_block - 1) / per_block + 1) * 2 * kGMaxOps * sizeof(double);
+}
+
 AdjScratch &adj_scratch()
 {
     static AdjScratch *a = new AdjScratch();  // lives until the process ends
@@ -410,6 +494,16 @@ AdjScratch &adj_scratch()
 }
 
 }  // namespace
+
+// Size the adjoint's scratch for circuits of up to n qubits and n_ops ops, so
+// that later calls allocate (and free) nothing.
+int adjoint_reserve(int n, int prec, uint64_t n_ops)
+{
+    AdjScratch &sc = adj_scratch();
+    std::lock_guard<std::mutex> lock(sc.mu);
+    SPZ_TRY(sc.reserve_states(n, prec));
+    return sc.buffers(grad_bytes(n_ops), part_bytes(1ull << n));
+}
 
 int adjoint_grad(spz_state_s *s, const spz_op *ops, uint64_t n_ops, double hx, int ckpt_every,
                  double *grad, double *energy, double *drift)
@@ -457,28 +551,26 @@ int adjoint_grad(spz_state_s *s, const spz_op *ops, uint64_t n_ops, double hx, i
     AdjScratch &sc = adj_scratch();
     std::lock_guard<std::mutex> lock(sc.mu);
     SPZ_TRY(sc.states_like(s));
-    OwnedState &work = sc.work, &lam = sc.lam;
-    SPZ_TRY(spz_state_copy(work.s, s));
+    spz_state_s *const wk = &sc.work, *const lm = &sc.lam;  // psi copy, lambda
+    SPZ_TRY(spz_state_copy(wk, s));
     {
         LaunchTimer tm(SPZ_K_ADJOINT, (double)s->N * (s->prec == SPZ_F64 ? 16.0 : 8.0) * (n + 2));
         if (s->prec == SPZ_F64)
             hipLaunchKernelGGL((k_ising_apply<double>), dim3(blocks_for(s->N, kT)), dim3(kT), 0,
                                rt().stream, (const double *)s->re, (const double *)s->im,
-                               (double *)lam.s->re, (double *)lam.s->im, s->N, n, hx);
+                               (double *)lm->re, (double *)lm->im, s->N, n, hx);
         else
             hipLaunchKernelGGL((k_ising_apply<float>), dim3(blocks_for(s->N, kT)), dim3(kT), 0,
                                rt().stream, (const float *)s->re, (const float *)s->im,
-                               (float *)lam.s->re, (float *)lam.s->im, s->N, n, hx);
+                               (float *)lm->re, (float *)lm->im, s->N, n, hx);
         SPZ_HIP(hipGetLastError());
     }
     double e = 0.0;
-    SPZ_TRY(reduce(s, lam.s, TERM_DOT_RE, 0, 0, 0.0, &e));
+    SPZ_TRY(reduce(s, lm, TERM_DOT_RE, 0, 0, 0.0, &e));
     if (energy) *energy = e;
     // ---- reverse sweep -------------------------------------------------------
     const uint64_t per_block = (uint64_t)kT * kGPerThread;
-    const uint64_t max_blocks = (s->N / 2 + per_block - 1) / per_block + 1;
-    SPZ_TRY(sc.buffers(std::max<uint64_t>(n_ops, 1) * sizeof(double),
-                       max_blocks * 2 * kGMaxOps * sizeof(double)));
+    SPZ_TRY(sc.buffers(grad_bytes(n_ops), part_bytes(s->N)));
     SPZ_HIP(hipMemsetAsync(sc.grad, 0, std::max<uint64_t>(n_ops, 1) * sizeof(double), rt().stream));
     SPZ_HIP(hipMemsetAsync(sc.dmax, 0, sizeof(unsigned long long), rt().stream));
     double *gdev = (double *)sc.grad, *part = (double *)sc.part;
@@ -503,11 +595,11 @@ int adjoint_grad(spz_state_s *s, const spz_op *ops, uint64_t n_ops, double hx, i
             std::vector<spz_op> inv;
             inv.reserve(hi + 1 - lo);
             for (uint64_t k = hi + 1; k-- > lo;) inv.push_back(inverse_op(ops[k]));
-            SPZ_TRY(execute_ops(work.s, inv.data(), inv.size(), nullptr, &eo, nullptr));
-            SPZ_TRY(execute_ops(lam.s, inv.data(), inv.size(), nullptr, &eo, nullptr));
+            SPZ_TRY(execute_ops(wk, inv.data(), inv.size(), nullptr, &eo, nullptr));
+            SPZ_TRY(execute_ops(lm, inv.data(), inv.size(), nullptr, &eo, nullptr));
         } else if (o.kind == SPZ_SWAP) {  // self-inverse, no parameter
-            SPZ_TRY(apply_gate(work.s, SPZ_SWAP, 0, 0, 0, o.target, o.t1, o.ones, o.zeros));
-            SPZ_TRY(apply_gate(lam.s, SPZ_SWAP, 0, 0, 0, o.target, o.t1, o.ones, o.zeros));
+            SPZ_TRY(apply_gate(wk, SPZ_SWAP, 0, 0, 0, o.target, o.t1, o.ones, o.zeros));
+            SPZ_TRY(apply_gate(lm, SPZ_SWAP, 0, 0, 0, o.target, o.t1, o.ones, o.zeros));
         } else {
             // greedy group: ops hi, hi-1, ... while the targets fit in kGMaxQ qubits
             GroupDesc gd;
@@ -534,6 +626,8 @@ int adjoint_grad(spz_state_s *s, const spz_op *ops, uint64_t n_ops, double hx, i
                 Coef kc;
                 SPZ_TRY(make_coef(g.kind, th, ph, la, &kc));
                 go.ak = kc.ak;
+                go.kb = kind_bit_of(kc.ak);
+                if (!go.kb) return fail(SPZ_ERR_GATE, "adjoint: unsupported arithmetic");
                 memcpy(go.c, kc.c, sizeof go.c);
                 go.gk = grad_kind(g.kind);
                 for (int b = 0; b < nq; b++)
@@ -557,7 +651,7 @@ int adjoint_grad(spz_state_s *s, const spz_op *ops, uint64_t n_ops, double hx, i
             const unsigned g = blocks_for(nel, per_block);
             {
                 LaunchTimer tm(SPZ_K_ADJOINT, 2.0 * 2.0 * (double)(nel << nq) * amp);
-                SPZ_TRY(launch_group(work.s, lam.s, nq, nel, pl, gd, part, g));
+                SPZ_TRY(launch_group(wk, lm, nq, nel, pl, gd, part, g));
             }
             if (ngrad) {  // one block per op; ops without a gradient write nothing
                 hipLaunchKernelGGL(k_adj_finish, dim3(gd.nop), dim3(kT), 0, rt().stream, part,
@@ -570,16 +664,16 @@ int adjoint_grad(spz_state_s *s, const spz_op *ops, uint64_t n_ops, double hx, i
             spz_state_s *ck = ckpts[next_ck]->s;
             if (s->prec == SPZ_F64)
                 hipLaunchKernelGGL((k_maxdiff<double>), dim3(maxdiff_grid(s->N)), dim3(kT), 0,
-                                   rt().stream, (const double *)work.s->re,
-                                   (const double *)work.s->im, (const double *)ck->re,
+                                   rt().stream, (const double *)wk->re,
+                                   (const double *)wk->im, (const double *)ck->re,
                                    (const double *)ck->im, s->N, (unsigned long long *)dmax_p);
             else
                 hipLaunchKernelGGL((k_maxdiff<float>), dim3(maxdiff_grid(s->N)), dim3(kT), 0,
-                                   rt().stream, (const float *)work.s->re,
-                                   (const float *)work.s->im, (const float *)ck->re,
+                                   rt().stream, (const float *)wk->re,
+                                   (const float *)wk->im, (const float *)ck->re,
                                    (const float *)ck->im, s->N, (unsigned long long *)dmax_p);
             SPZ_HIP(hipGetLastError());
-            SPZ_TRY(spz_state_copy(work.s, ck));
+            SPZ_TRY(spz_state_copy(wk, ck));
             next_ck--;
         }
         ii = lo;

@@ -19,6 +19,7 @@
 // The instructions and their order are those of pair_update/diag_update (the
 // reference's per-gate arithmetic, FMA exactly where it writes mul_add), so a
 // sweep that keeps program order is bit-identical to one launch per gate.
+#include "spz_inplace.h"
 #include "spz_internal.h"
 #include "spz_tile.h"
 
@@ -34,169 +35,6 @@ namespace {
 // per-lane term is formed once per layout change and the per-register term is
 // wavefront-uniform (scalar), leaving one vector XOR per LDS address.
 __device__ __forceinline__ uint32_t swz(uint32_t l) { return l ^ ((l >> 4) & 15u) ^ ((l >> 8) & 15u); }
-
-// ---------------------------------------------------------------------------
-// In-place pair arithmetic.  (a, b) = psi0 (re, im), (c, d) = psi1.
-// ---------------------------------------------------------------------------
-
-// H (gates.rs:547-560): a1 = h a, c1 = h c, a = a1 + c1, c = a1 - c1; same for b, d.
-__device__ __forceinline__ void ip_h(double h, double &a, double &b, double &c, double &d)
-{
-    double t0, t1;
-    asm volatile(
-        "v_mul_f64 %0, %6, %2\n\t"
-        "v_mul_f64 %1, %6, %4\n\t"
-        "v_add_f64 %2, %0, %1\n\t"
-        "v_add_f64 %4, %0, -%1\n\t"
-        "v_mul_f64 %0, %6, %3\n\t"
-        "v_mul_f64 %1, %6, %5\n\t"
-        "v_add_f64 %3, %0, %1\n\t"
-        "v_add_f64 %5, %0, -%1"
-        : "=&v"(t0), "=&v"(t1), "+v"(a), "+v"(b), "+v"(c), "+v"(d)
-        : "s"(h));
-}
-
-// X (gates.rs:329-334): swap psi0 and psi1.
-__device__ __forceinline__ void ip_swap(double &x, double &y)
-{
-    double t;
-    asm volatile("v_mov_b64 %2, %0\n\tv_mov_b64 %0, %1\n\tv_mov_b64 %1, %2"
-                 : "+v"(x), "+v"(y), "=&v"(t));
-}
-
-// Y (gates.rs:455-464): (a, b, c, d) <- (d, -c, -b, a); negation is a multiply
-// by -1 (exact, and it flips the sign of zero like -x).
-__device__ __forceinline__ void ip_y(double &a, double &b, double &c, double &d)
-{
-    double t;
-    asm volatile(
-        "v_mov_b64 %4, %0\n\t"
-        "v_mov_b64 %0, %3\n\t"
-        "v_mov_b64 %3, %4\n\t"
-        "v_mov_b64 %4, %1\n\t"
-        "v_mul_f64 %1, -1.0, %2\n\t"
-        "v_mul_f64 %2, -1.0, %4"
-        : "+v"(a), "+v"(b), "+v"(c), "+v"(d), "=&v"(t));
-}
-
-// RX (gates.rs:721-724): a = a cs - d ns, b = b cs + c ns, c = b (-ns) + c cs,
-// d = d cs + a ns, with (cs, ns) = (cos, -sin) of theta/2.
-__device__ __forceinline__ void ip_rx(double cs, double ns, double &a, double &b, double &c,
-                                      double &d)
-{
-    double t0, t1, t2, t3;
-    asm volatile(
-        "v_mul_f64 %0, %4, %8\n\t"   // a cs
-        "v_mul_f64 %1, %7, %9\n\t"   // d ns
-        "v_mul_f64 %2, %7, %8\n\t"   // d cs
-        "v_mul_f64 %3, %4, %9\n\t"   // a ns
-        "v_add_f64 %4, %0, -%1\n\t"
-        "v_add_f64 %7, %2, %3\n\t"
-        "v_mul_f64 %0, %5, %8\n\t"   // b cs
-        "v_mul_f64 %1, %6, %9\n\t"   // c ns
-        "v_mul_f64 %2, %5, -%9\n\t"  // b (-ns)
-        "v_mul_f64 %3, %6, %8\n\t"   // c cs
-        "v_add_f64 %5, %0, %1\n\t"
-        "v_add_f64 %6, %2, %3"
-        : "=&v"(t0), "=&v"(t1), "=&v"(t2), "=&v"(t3), "+v"(a), "+v"(b), "+v"(c), "+v"(d)
-        : "s"(cs), "s"(ns));
-}
-
-// RY (gates.rs:1116-1120): a = a cs - c sn, b = b cs - d sn, c = a sn + c cs,
-// d = b sn + d cs.
-__device__ __forceinline__ void ip_ry(double sn, double cs, double &a, double &b, double &c,
-                                      double &d)
-{
-    double t0, t1, t2, t3;
-    asm volatile(
-        "v_mul_f64 %0, %4, %9\n\t"   // a cs
-        "v_mul_f64 %1, %6, %8\n\t"   // c sn
-        "v_mul_f64 %2, %4, %8\n\t"   // a sn
-        "v_mul_f64 %3, %6, %9\n\t"   // c cs
-        "v_add_f64 %4, %0, -%1\n\t"
-        "v_add_f64 %6, %2, %3\n\t"
-        "v_mul_f64 %0, %5, %9\n\t"   // b cs
-        "v_mul_f64 %1, %7, %8\n\t"   // d sn
-        "v_mul_f64 %2, %5, %8\n\t"   // b sn
-        "v_mul_f64 %3, %7, %9\n\t"   // d cs
-        "v_add_f64 %5, %0, -%1\n\t"
-        "v_add_f64 %7, %2, %3"
-        : "=&v"(t0), "=&v"(t1), "=&v"(t2), "=&v"(t3), "+v"(a), "+v"(b), "+v"(c), "+v"(d)
-        : "s"(sn), "s"(cs));
-}
-
-// U (gates.rs:1248-1266), with (cc, dd, m, n) = (a, b, c, d):
-//   a = fma(g, cc, fma(k, m, -l n))          b = fma(g, dd, fma(k, n, l m))
-//   c = fma(q, cc, fma(-r, dd, fma(s, m, -t n)))
-//   d = fma(q, dd, fma(r, cc, fma(s, n, t m)))
-// c and d are formed first (they need all four inputs), then a and b in place.
-__device__ __forceinline__ void ip_u(const double *k, double &a, double &b, double &c, double &d)
-{
-    double t0, t1, x;
-    asm volatile(
-        "v_mul_f64 %0, -%13, %6\n\t"     // -t n
-        "v_fma_f64 %0, %12, %5, %0\n\t"  // fma(s, m, .)
-        "v_fma_f64 %0, -%11, %4, %0\n\t" // fma(-r, dd, .)
-        "v_fma_f64 %0, %10, %3, %0\n\t"  // c' = fma(q, cc, .)
-        "v_mul_f64 %1, %13, %5\n\t"      // t m
-        "v_fma_f64 %1, %12, %6, %1\n\t"  // fma(s, n, .)
-        "v_fma_f64 %1, %11, %3, %1\n\t"  // fma(r, cc, .)
-        "v_fma_f64 %1, %10, %4, %1\n\t"  // d' = fma(q, dd, .)
-        "v_mul_f64 %2, -%9, %6\n\t"      // -l n
-        "v_fma_f64 %2, %8, %5, %2\n\t"   // fma(k, m, .)
-        "v_fma_f64 %3, %7, %3, %2\n\t"   // a' = fma(g, cc, .)
-        "v_mul_f64 %2, %9, %5\n\t"       // l m
-        "v_fma_f64 %2, %8, %6, %2\n\t"   // fma(k, n, .)
-        "v_fma_f64 %4, %7, %4, %2\n\t"   // b' = fma(g, dd, .)
-        "v_mov_b64 %5, %0\n\t"
-        "v_mov_b64 %6, %1"
-        : "=&v"(t0), "=&v"(t1), "=&v"(x), "+v"(a), "+v"(b), "+v"(c), "+v"(d)
-        : "s"(k[0]), "s"(k[2]), "s"(k[3]), "s"(k[4]), "s"(k[5]), "s"(k[6]), "s"(k[7]));
-}
-
-// RZ multiply in place (gates.rs:940-946): x = fma(x, mr, -y mi),
-// y = fma(x, mi, y mr); y is formed first, while x still holds its input.
-template <bool VCOEF>
-__device__ __forceinline__ void ip_rz(double mr, double mi, double &x, double &y)
-{
-    double t0, t1;
-    if constexpr (VCOEF)
-        asm volatile(
-            "v_mul_f64 %0, %3, %5\n\t"
-            "v_mul_f64 %1, %3, %4\n\t"
-            "v_fma_f64 %3, %2, %5, %1\n\t"
-            "v_fma_f64 %2, %2, %4, -%0"
-            : "=&v"(t0), "=&v"(t1), "+v"(x), "+v"(y)
-            : "v"(mr), "v"(mi));
-    else
-        asm volatile(
-            "v_mul_f64 %0, %3, %5\n\t"
-            "v_mul_f64 %1, %3, %4\n\t"
-            "v_fma_f64 %3, %2, %5, %1\n\t"
-            "v_fma_f64 %2, %2, %4, -%0"
-            : "=&v"(t0), "=&v"(t1), "+v"(x), "+v"(y)
-            : "s"(mr), "s"(mi));
-}
-
-// P multiply in place (gates.rs:835-838): x = fma(x, mr, -y mi),
-// y = fma(y, mr, x mi).
-__device__ __forceinline__ void ip_phase(double mr, double mi, double &x, double &y)
-{
-    double t0, t1;
-    asm volatile(
-        "v_mul_f64 %0, %3, %5\n\t"
-        "v_mul_f64 %1, %2, %5\n\t"
-        "v_fma_f64 %3, %3, %4, %1\n\t"
-        "v_fma_f64 %2, %2, %4, -%0"
-        : "=&v"(t0), "=&v"(t1), "+v"(x), "+v"(y)
-        : "s"(mr), "s"(mi));
-}
-
-// (x, y) <- (x sg, y sg) for a per-lane sg of +1 or -1.
-__device__ __forceinline__ void ip_scale(double sg, double &x, double &y)
-{
-    asm volatile("v_mul_f64 %0, %0, %2\n\tv_mul_f64 %1, %1, %2" : "+v"(x), "+v"(y) : "v"(sg));
-}
 
 template <int AK>
 __device__ __forceinline__ void ip_pair(const double *k, double &a, double &b, double &c, double &d)
