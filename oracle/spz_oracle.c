@@ -26,6 +26,15 @@
  *     restated here operation for operation so GPU results can be compared
  *     bit for bit.
  *
+ * Parity unpinned: the reference draws measurement outcomes and reservoir
+ * samples from an unseeded thread_rng (measurement.rs:35-36 through
+ * rand_distr ^0.4.3 Binomial(1, p); core.rs:81-112 through rand ^0.8.5), so no
+ * seeded value of the reference exists to compare with.  orc_measure_qubit and
+ * orc_sample restate the build's own rules (outcome 1 iff u < 1 - prob0;
+ * inverse-CDF over fixed-tree block sums); they are pinned by the forced-outcome
+ * known-answer test, closed forms, brute-force inverse CDF on small states and
+ * a chi-squared test of the sampler, not by the reference.
+ *
  * Compile with -ffp-contract=off (no implicit FMA) and without -ffast-math:
  * every rounding step below is intentional.
  *
