@@ -266,10 +266,13 @@ struct Planner {
         memset(&op, 0, sizeof op);
         op.type = TOP_LAYOUT;
         op.grp = TG_LAYOUT;
+        op.eo = 1;  // em = 0: never taken for a gate
         uint32_t used = 0;
         for (int j = 0; j < RB; j++) {
             op.lay.oldreg[j] = (int8_t)oldRS[j];
             op.lay.newreg[j] = (int8_t)RSv[j];
+            op.lay.rob[j] = tile_swz(1u << oldRS[j]) << 3;
+            op.lay.rnb[j] = tile_swz(1u << RSv[j]) << 3;
             used |= 1u << RSv[j];
         }
         int i = 0;
@@ -429,7 +432,8 @@ struct Planner {
                 if (!one && !zero) continue;
                 const int b = lbit[q];
                 if (b < 0) {
-                    if (one) op.eo |= 1ull << q; else op.ez |= 1ull << q;
+                    if (one) op.eo |= 1ull << q;
+                    op.em |= 1ull << q;
                     continue;
                 }
                 const int j = slot_of(b);

@@ -38,14 +38,30 @@ enum TileGroup : uint32_t {
     TG_RZ = 1u << 17,
 };
 
-struct TileLayout {
-    int8_t oldreg[8];   // local bit of register-index bit j before ...
-    int8_t newreg[8];   // ... and after the change
-    int8_t newthr[16];  // local bit of thread-index bit i after the change
-};
+// LDS bank swizzle for 8-byte elements of a tile: XOR the low 4 bits of the
+// local index with two higher nibbles so that power-of-two strides across
+// lanes spread over the banks (a bijection inside every aligned block of 16).
+// swz is linear over XOR, and the lane part and the register part of a local
+// index have disjoint bits, so swz(lane | reg) = swz(lane) ^ swz(reg).
+__host__ __device__ __forceinline__ uint32_t tile_swz(uint32_t l)
+{
+    return l ^ ((l >> 4) & 15u) ^ ((l >> 8) & 15u);
+}
 
-// 128 bytes: fetched whole into scalar registers, the next one prefetched
-// while the current one executes.
+// A change of register layout.  The register part of an LDS byte offset is
+// linear in the register index, so the host hands over its value on each
+// register-index bit (rob/rnb) and the kernel forms the 2^RB offsets with one
+// scalar XOR each.
+struct TileLayout {
+    int8_t newthr[16];  // local bit of thread-index bit i after the change
+    uint32_t rob[4];    // 8 * tile_swz(1 << local bit of register-index bit j) before ...
+    uint32_t rnb[4];    // ... and after the change
+    int8_t oldreg[8];   // local bit of register-index bit j before ...
+    int8_t newreg[8];   // ... and after the change (host bookkeeping)
+};
+constexpr int kTileLayoutRB = 4;  // register-index bits a TileLayout describes
+
+// 128 bytes, fetched whole into scalar registers by each wavefront.
 struct TileOp {
     int32_t type;     // TileOpType
     int32_t ak;       // Arith kind
@@ -57,7 +73,8 @@ struct TileOp {
     uint32_t pad;
     uint32_t mlo, mlz;  // controls on local bits held by the lane part (per lane)
     uint32_t grp;       // TileGroup bits
-    uint64_t eo, ez;    // controls outside S (global index bits), uniform per tile
+    uint64_t eo, em;    // controls outside S: the op runs on a tile iff (base & em) == eo
+                        // (a layout change has em = 0, eo = 1 and never runs as a gate)
     uint64_t ebit;      // DIAG with the target outside S: its global bit
     union {
         double c[8];    // PAIR / DIAG coefficients

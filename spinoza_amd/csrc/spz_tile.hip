@@ -27,14 +27,10 @@ namespace spz {
 
 namespace {
 
-// LDS bank swizzle for 8-byte elements: XOR the low 4 bits of the element
-// index with two higher nibbles so that power-of-two strides across lanes
-// spread over the banks (a bijection inside every aligned block of 16).
-// swz is linear over XOR, and the lane part and the register part of a local
-// index have disjoint bits, so swz(lane | reg) = swz(lane) ^ swz(reg): the
-// per-lane term is formed once per layout change and the per-register term is
+// The LDS swizzle (tile_swz) is linear, so the per-lane term of an LDS
+// address is formed once per layout change and the per-register term is
 // wavefront-uniform (scalar), leaving one vector XOR per LDS address.
-__device__ __forceinline__ uint32_t swz(uint32_t l) { return l ^ ((l >> 4) & 15u) ^ ((l >> 8) & 15u); }
+__device__ __forceinline__ uint32_t swz(uint32_t l) { return tile_swz(l); }
 
 template <int AK>
 __device__ __forceinline__ void ip_pair(const double *k, double &a, double &b, double &c, double &d)
@@ -51,10 +47,11 @@ __device__ __forceinline__ void ip_pair(const double *k, double &a, double &b, d
 // Dispatch (one-hot bits set by the host; see TileOp::grp)
 // ---------------------------------------------------------------------------
 
-// Pair update on register bit J.  PRED: controls on register bits (uniform
-// per register) and on lane bits (lane_ok, per lane).
+// Pair update on register bit J.  PRED: controls on register bits, as the
+// host-computed mask pm of the registers that meet them (uniform); controls
+// on lane bits are one divergent branch around the whole op (see k_tile).
 template <int R, int AK, int J, bool PRED>
-__device__ __forceinline__ void pair_j(double *vr, double *vi, const double *k, uint32_t pm, bool lane_ok)
+__device__ __forceinline__ void pair_j(double *vr, double *vi, const double *k, uint32_t pm)
 {
 #pragma unroll
     for (int p = 0; p < R; p++) {
@@ -71,25 +68,24 @@ __device__ __forceinline__ void pair_j(double *vr, double *vi, const double *k, 
 
 template <int R, int RB, int AK, bool PRED>
 __device__ __forceinline__ void pair_sel(uint32_t sel, double *vr, double *vi, const double *k,
-                                         uint32_t pm, bool lane_ok)
+                                         uint32_t pm)
 {
-    if (sel & 1u) pair_j<R, AK, 0, PRED>(vr, vi, k, pm, lane_ok);
-    if constexpr (RB > 1) if (sel & 2u) pair_j<R, AK, 1, PRED>(vr, vi, k, pm, lane_ok);
-    if constexpr (RB > 2) if (sel & 4u) pair_j<R, AK, 2, PRED>(vr, vi, k, pm, lane_ok);
-    if constexpr (RB > 3) if (sel & 8u) pair_j<R, AK, 3, PRED>(vr, vi, k, pm, lane_ok);
+    if (sel & 1u) pair_j<R, AK, 0, PRED>(vr, vi, k, pm);
+    if constexpr (RB > 1) if (sel & 2u) pair_j<R, AK, 1, PRED>(vr, vi, k, pm);
+    if constexpr (RB > 2) if (sel & 4u) pair_j<R, AK, 2, PRED>(vr, vi, k, pm);
+    if constexpr (RB > 3) if (sel & 8u) pair_j<R, AK, 3, PRED>(vr, vi, k, pm);
 }
 
 template <int R, int RB, bool PRED>
 __device__ __forceinline__ void pair_kinds(uint32_t g, uint32_t sel, double *vr, double *vi,
-                                           const double *k, uint32_t pm,
-                                           bool lane_ok)
+                                           const double *k, uint32_t pm)
 {
-    if (g & TG_H) pair_sel<R, RB, AK_H, PRED>(sel, vr, vi, k, pm, lane_ok);
-    if (g & TG_X) pair_sel<R, RB, AK_X, PRED>(sel, vr, vi, k, pm, lane_ok);
-    if (g & TG_Y) pair_sel<R, RB, AK_Y, PRED>(sel, vr, vi, k, pm, lane_ok);
-    if (g & TG_RX) pair_sel<R, RB, AK_RX, PRED>(sel, vr, vi, k, pm, lane_ok);
-    if (g & TG_RY) pair_sel<R, RB, AK_RY, PRED>(sel, vr, vi, k, pm, lane_ok);
-    if (g & TG_U) pair_sel<R, RB, AK_U, PRED>(sel, vr, vi, k, pm, lane_ok);
+    if (g & TG_H) pair_sel<R, RB, AK_H, PRED>(sel, vr, vi, k, pm);
+    if (g & TG_X) pair_sel<R, RB, AK_X, PRED>(sel, vr, vi, k, pm);
+    if (g & TG_Y) pair_sel<R, RB, AK_Y, PRED>(sel, vr, vi, k, pm);
+    if (g & TG_RX) pair_sel<R, RB, AK_RX, PRED>(sel, vr, vi, k, pm);
+    if (g & TG_RY) pair_sel<R, RB, AK_RY, PRED>(sel, vr, vi, k, pm);
+    if (g & TG_U) pair_sel<R, RB, AK_U, PRED>(sel, vr, vi, k, pm);
 }
 
 // Diagonal update of one amplitude whose target bit is known at compile
@@ -110,7 +106,7 @@ __device__ __forceinline__ void diag_known(const double *k, bool bit, double &x,
 
 // Diagonal target on register bit J.
 template <int R, int AK, int J, bool PRED>
-__device__ __forceinline__ void diag_j(double *vr, double *vi, const double *k, uint32_t pm, bool lane_ok)
+__device__ __forceinline__ void diag_j(double *vr, double *vi, const double *k, uint32_t pm)
 {
 #pragma unroll
     for (int p = 0; p < R; p++) {
@@ -128,19 +124,19 @@ __device__ __forceinline__ void diag_j(double *vr, double *vi, const double *k, 
 
 template <int R, int RB, int AK, bool PRED>
 __device__ __forceinline__ void diag_sel(uint32_t sel, double *vr, double *vi, const double *k,
-                                         uint32_t pm, bool lane_ok)
+                                         uint32_t pm)
 {
-    if (sel & 1u) diag_j<R, AK, 0, PRED>(vr, vi, k, pm, lane_ok);
-    if constexpr (RB > 1) if (sel & 2u) diag_j<R, AK, 1, PRED>(vr, vi, k, pm, lane_ok);
-    if constexpr (RB > 2) if (sel & 4u) diag_j<R, AK, 2, PRED>(vr, vi, k, pm, lane_ok);
-    if constexpr (RB > 3) if (sel & 8u) diag_j<R, AK, 3, PRED>(vr, vi, k, pm, lane_ok);
+    if (sel & 1u) diag_j<R, AK, 0, PRED>(vr, vi, k, pm);
+    if constexpr (RB > 1) if (sel & 2u) diag_j<R, AK, 1, PRED>(vr, vi, k, pm);
+    if constexpr (RB > 2) if (sel & 4u) diag_j<R, AK, 2, PRED>(vr, vi, k, pm);
+    if constexpr (RB > 3) if (sel & 8u) diag_j<R, AK, 3, PRED>(vr, vi, k, pm);
 }
 
 // Diagonal target on a lane bit or outside the tile: `bit` per lane (or per
 // tile); the RZ multipliers are selected once per op.
 template <int R, int AK, bool PRED>
 __device__ __forceinline__ void diag_lane(double *vr, double *vi, const double *k, bool bit,
-                                          uint32_t pm, bool lane_ok)
+                                          uint32_t pm)
 {
     if constexpr (AK == AK_RZ) {
         const double mr = bit ? k[2] : k[0];
@@ -181,32 +177,20 @@ __device__ __forceinline__ void diag_lane(double *vr, double *vi, const double *
 
 template <int R, int RB, bool PRED>
 __device__ __forceinline__ void diag_reg_kinds(uint32_t g, uint32_t sel, double *vr, double *vi,
-                                               const double *k, uint32_t pm,
-                                               bool lane_ok)
+                                               const double *k, uint32_t pm)
 {
-    if (g & TG_NEG) diag_sel<R, RB, AK_NEG, PRED>(sel, vr, vi, k, pm, lane_ok);
-    if (g & TG_PHASE) diag_sel<R, RB, AK_PHASE, PRED>(sel, vr, vi, k, pm, lane_ok);
-    if (g & TG_RZ) diag_sel<R, RB, AK_RZ, PRED>(sel, vr, vi, k, pm, lane_ok);
+    if (g & TG_NEG) diag_sel<R, RB, AK_NEG, PRED>(sel, vr, vi, k, pm);
+    if (g & TG_PHASE) diag_sel<R, RB, AK_PHASE, PRED>(sel, vr, vi, k, pm);
+    if (g & TG_RZ) diag_sel<R, RB, AK_RZ, PRED>(sel, vr, vi, k, pm);
 }
 
 template <int R, bool PRED>
 __device__ __forceinline__ void diag_lane_kinds(uint32_t g, double *vr, double *vi,
-                                                const double *k, bool bit, uint32_t pm, bool lane_ok)
+                                                const double *k, bool bit, uint32_t pm)
 {
-    if (g & TG_NEG) diag_lane<R, AK_NEG, PRED>(vr, vi, k, bit, pm, lane_ok);
-    if (g & TG_PHASE) diag_lane<R, AK_PHASE, PRED>(vr, vi, k, bit, pm, lane_ok);
-    if (g & TG_RZ) diag_lane<R, AK_RZ, PRED>(vr, vi, k, bit, pm, lane_ok);
-}
-
-// Register part of the local index of register p under a layout.
-template <int RB>
-__device__ __forceinline__ uint32_t reg_part(int p, const int8_t *regbits)
-{
-    uint32_t r = 0;
-#pragma unroll
-    for (int j = 0; j < RB; j++)
-        if ((p >> j) & 1) r |= 1u << regbits[j];
-    return r;
+    if (g & TG_NEG) diag_lane<R, AK_NEG, PRED>(vr, vi, k, bit, pm);
+    if (g & TG_PHASE) diag_lane<R, AK_PHASE, PRED>(vr, vi, k, bit, pm);
+    if (g & TG_RZ) diag_lane<R, AK_RZ, PRED>(vr, vi, k, bit, pm);
 }
 
 // Register budget: asking for 6 waves per SIMD makes the compiler fit the
@@ -219,6 +203,7 @@ k_tile(T *__restrict__ re, T *__restrict__ im, const TileDesc d, const TileOp *_
 {
     constexpr int RB = K - LOGW;
     constexpr int R = 1 << RB;
+    static_assert(RB <= kTileLayoutRB, "a layout change describes at most 4 register-index bits");
     __shared__ double lds[1 << K];
     const uint32_t tid = threadIdx.x;
 
@@ -244,22 +229,29 @@ k_tile(T *__restrict__ re, T *__restrict__ im, const TileDesc d, const TileOp *_
     for (int o = 0; o < nops; o++) {
         // by value: the op's fields come into scalar registers, so selects
         // between its coefficients are value selects, never per-lane loads
-        const TileOp cur = ops[o];
+        // (the opaque index keeps every field at an immediate offset of one
+        // scalar address instead of strength-reduced bases per field)
+        int oi = o;
+        asm volatile("" : "+s"(oi));
+        const TileOp cur = ops[oi];
         const uint32_t g = cur.grp;
         if (g & TG_LAYOUT) {
             uint32_t nlt = 0;
 #pragma unroll
-            for (int i = 0; i < LOGW; i++)
-                if ((tid >> i) & 1u) nlt |= 1u << cur.lay.newthr[i];
+            for (int i = 0; i < LOGW; i++) nlt |= ((tid >> i) & 1u) << cur.lay.newthr[i];
             // byte offsets into lds: lane term per lane, register term uniform
             uint32_t so = swz(lt) << 3, sn = swz(nlt) << 3;
             // (keeps the compiler from factoring the shift back out of the XORs)
             asm volatile("" : "+v"(so), "+v"(sn));
+            // register terms: linear in p, one XOR from p without its lowest bit
             uint32_t ro[R], rn[R];
+            ro[0] = 0;
+            rn[0] = 0;
 #pragma unroll
-            for (int p = 0; p < R; p++) {
-                ro[p] = swz(reg_part<RB>(p, cur.lay.oldreg)) << 3;
-                rn[p] = swz(reg_part<RB>(p, cur.lay.newreg)) << 3;
+            for (int p = 1; p < R; p++) {
+                const int j = __builtin_ctz(p);
+                ro[p] = ro[p & (p - 1)] ^ cur.lay.rob[j];
+                rn[p] = rn[p & (p - 1)] ^ cur.lay.rnb[j];
             }
             char *lb = reinterpret_cast<char *>(lds);
             __syncthreads();
@@ -277,26 +269,26 @@ k_tile(T *__restrict__ re, T *__restrict__ im, const TileDesc d, const TileOp *_
             lt = nlt;
         }
         // (a control outside the tile that is not met skips the op)
-        if ((g & TG_GATE) && (base & cur.eo) == cur.eo && !(base & cur.ez)) {
+        if ((base & cur.em) == cur.eo) {
             const uint32_t sel = cur.sel, pm = cur.pm;
             // controls on lane bits: a per-lane predicate inside the predicated paths
             const bool lane_ok = (lt & cur.mlo) == cur.mlo && !(lt & cur.mlz);
             const double *k = cur.c;
-            if (g & TG_PAIR) pair_kinds<R, RB, false>(g, sel, vr, vi, k, 0, true);
+            if (g & TG_PAIR) pair_kinds<R, RB, false>(g, sel, vr, vi, k, 0);
             // predicated ops: the lane part of the controls is one divergent
             // branch around the op, the register part a uniform test per register
             if (g & TG_PAIRP)
-                if (lane_ok) pair_kinds<R, RB, true>(g, sel, vr, vi, k, pm, true);
-            if (g & TG_DREG) diag_reg_kinds<R, RB, false>(g, sel, vr, vi, k, 0, true);
+                if (lane_ok) pair_kinds<R, RB, true>(g, sel, vr, vi, k, pm);
+            if (g & TG_DREG) diag_reg_kinds<R, RB, false>(g, sel, vr, vi, k, 0);
             if (g & TG_DREGP)
-                if (lane_ok) diag_reg_kinds<R, RB, true>(g, sel, vr, vi, k, pm, true);
+                if (lane_ok) diag_reg_kinds<R, RB, true>(g, sel, vr, vi, k, pm);
             if (g & (TG_DLANE | TG_DLANEP)) {
                 // target on a lane bit (dsel 1) or outside the tile (dsel 2)
                 const bool bit = cur.dsel == 2 ? (base & cur.ebit) != 0
                                                : ((lt >> cur.dpos) & 1u) != 0;
-                if (g & TG_DLANE) diag_lane_kinds<R, false>(g, vr, vi, k, bit, 0, true);
+                if (g & TG_DLANE) diag_lane_kinds<R, false>(g, vr, vi, k, bit, 0);
                 if (g & TG_DLANEP)
-                    if (lane_ok) diag_lane_kinds<R, true>(g, vr, vi, k, bit, pm, true);
+                    if (lane_ok) diag_lane_kinds<R, true>(g, vr, vi, k, bit, pm);
             }
         }
     }
