@@ -285,9 +285,13 @@ struct Planner {
     // with reordering allowed, a topological order of the dependency graph
     // that applies every gate the current register layout allows before
     // switching layouts.  Entries >= 0 are gates; -1 marks "switch layout to
-    // the next entry of `layouts`".
+    // the next entry of `layouts`".  rs_in receives the register bits of the
+    // load layout: never one of the klo low local bits (they stay on the low
+    // lane bits, for contiguous loads); with reordering allowed, the bits
+    // with the most ready pair gates.
     void sequence(const std::vector<SG> &run, const std::vector<int> &order, const int *lbit,
-                  std::vector<int> &seq, std::vector<std::vector<int>> &layouts)
+                  std::vector<int> &seq, std::vector<std::vector<int>> &layouts,
+                  std::vector<int> &rs_in)
     {
         const int m = (int)order.size();
         std::vector<int> RS(RB);
@@ -297,6 +301,29 @@ struct Planner {
                 if (RS[j] == b) return true;
             return false;
         };
+        if (!strict) {
+            std::vector<long> score(K, 0);
+            {
+                std::vector<int> deps(m, 0);
+                for (int a = 0; a < m; a++) {
+                    const SG &x = run[order[a]];
+                    for (int b = a + 1; b < m; b++) {
+                        const SG &y = run[order[b]];
+                        if ((x.nd & (y.nd | y.dm)) || (x.dm & y.nd)) deps[b]++;
+                    }
+                }
+                for (int a = 0; a < m; a++) {
+                    const SG &g = run[order[a]];
+                    if (!g.diag) score[lbit[g.target]] += deps[a] == 0 ? 1000 : 1;
+                }
+            }
+            std::vector<int> bits;
+            for (int b = klo; b < K; b++) bits.push_back(b);
+            std::stable_sort(bits.begin(), bits.end(),
+                             [&](int x, int y) { return score[x] > score[y]; });
+            if ((int)bits.size() >= RB) RS.assign(bits.begin(), bits.begin() + RB);
+        }
+        rs_in = RS;
         if (strict) {
             for (int pos = 0; pos < m; pos++) {
                 const SG &g = run[order[pos]];
@@ -317,6 +344,17 @@ struct Planner {
                     layouts.push_back(RS);
                 }
                 seq.push_back(order[pos]);
+            }
+            // a first layout change ahead of every gate that keeps the low
+            // bits on lanes becomes the load layout
+            if (!seq.empty() && seq[0] == -1) {
+                bool low = false;
+                for (int b : layouts[0]) low |= b < klo;
+                if (!low) {
+                    rs_in = layouts[0];
+                    seq.erase(seq.begin());
+                    layouts.erase(layouts.begin());
+                }
             }
             return;
         }
@@ -396,18 +434,27 @@ struct Planner {
                 lbit[q] = i++;
             }
         for (; i < 16; i++) sw.desc.spos[i] = 64;
-        for (int p = 0; p < (1 << RB); p++) {
-            uint64_t g = 0;
-            for (int j = 0; j < RB; j++)
-                if ((p >> j) & 1) g |= 1ull << sw.desc.spos[LOGW + j];
-            sw.desc.gr[p] = g;
-        }
         sw.op_begin = prog.size();
-        std::vector<int> seq;
+        std::vector<int> seq, RS;
         std::vector<std::vector<int>> layouts;
-        sequence(run, order, lbit, seq, layouts);
-        std::vector<int> L0(RB), RS(RB);
-        for (int j = 0; j < RB; j++) L0[j] = RS[j] = LOGW + j;
+        sequence(run, order, lbit, seq, layouts, RS);
+        // load layout: register bits RS, thread bits the other local bits ascending
+        auto reg_offsets = [&](const std::vector<int> &rs, uint64_t *out) {
+            for (int p = 0; p < (1 << RB); p++) {
+                uint64_t g = 0;
+                for (int j = 0; j < RB; j++)
+                    if ((p >> j) & 1) g |= 1ull << sw.desc.spos[rs[j]];
+                out[p] = g;
+            }
+        };
+        reg_offsets(RS, sw.desc.gr);
+        {
+            uint32_t used = 0;
+            for (int b : RS) used |= 1u << b;
+            int t = 0;
+            for (int b = 0; b < K; b++)
+                if (!((used >> b) & 1)) sw.desc.tin[t++] = (int8_t)b;
+        }
         size_t li_next = 0;
         for (int gi : seq) {
             if (gi < 0) {
@@ -481,7 +528,26 @@ struct Planner {
             }
             prog.push_back(op);
         }
-        if (RS != L0) emit_layout(RS, L0);
+        // store layout: the last one, unless it holds a low bit in registers;
+        // then those register bits are swapped for unused high bits
+        {
+            std::vector<int> st = RS;
+            uint32_t used = 0;
+            for (int b : RS) used |= 1u << b;
+            int nb = K - 1;
+            for (int j = 0; j < RB; j++) {
+                if (st[j] >= klo) continue;
+                while (nb >= klo && ((used >> nb) & 1)) nb--;
+                if (nb < klo) {  // too few high bits: the top RB local bits
+                    for (int i = 0; i < RB; i++) st[i] = LOGW + i;
+                    break;
+                }
+                st[j] = nb;
+                used |= 1u << nb;
+            }
+            if (st != RS) emit_layout(RS, st);
+            reg_offsets(st, sw.desc.gs);
+        }
         sw.op_count = prog.size() - sw.op_begin;
         sw.desc.nops = (int32_t)sw.op_count;
         sweeps.push_back(sw);

@@ -12,8 +12,11 @@
 // a gate whose target is a register bit is a pure in-register pair update,
 // with controls and diagonal phases evaluated from each amplitude's local
 // index (or, for qubits outside S, once per tile).  Layout changes go through
-// LDS.  Loads and stores use the load layout (register bits = top RB local
-// bits), so lanes of a wavefront read contiguous runs.
+// LDS.  The layouts a sweep loads in and stores from keep the low local bits
+// (the lowest qubits) on the low lane bits, so lanes of a wavefront read and
+// write contiguous runs; the register bits at load and store are chosen per
+// sweep, so that a sweep whose first or last layout meets that rule needs no
+// LDS pass to enter or leave it.
 #pragma once
 
 #include <stdint.h>
@@ -88,7 +91,10 @@ constexpr int kTileMaxR = 32;
 
 struct TileDesc {
     int8_t spos[16];          // qubit of local bit i (ascending), K entries
+    int8_t tin[16];           // local bit of thread-index bit i in the load layout
     uint64_t gr[kTileMaxR];   // global offset of register index p in the load layout
+    uint64_t gs[kTileMaxR];   // ... and in the store layout (its lane part follows the
+                              // program's last layout change)
     int32_t nops;
     int32_t pad;
 };

@@ -32,6 +32,10 @@ namespace {
 // wavefront-uniform (scalar), leaving one vector XOR per LDS address.
 __device__ __forceinline__ uint32_t swz(uint32_t l) { return tile_swz(l); }
 
+// Dispatch tests are marked unlikely, so each op's body sits out of line and
+// the tests that miss (nearly all of them, for every op) fall through untaken.
+__device__ __forceinline__ bool hit(uint32_t bits) { return __builtin_expect(bits != 0u, 0); }
+
 template <int AK>
 __device__ __forceinline__ void ip_pair(const double *k, double &a, double &b, double &c, double &d)
 {
@@ -70,22 +74,22 @@ template <int R, int RB, int AK, bool PRED>
 __device__ __forceinline__ void pair_sel(uint32_t sel, double *vr, double *vi, const double *k,
                                          uint32_t pm)
 {
-    if (sel & 1u) pair_j<R, AK, 0, PRED>(vr, vi, k, pm);
-    if constexpr (RB > 1) if (sel & 2u) pair_j<R, AK, 1, PRED>(vr, vi, k, pm);
-    if constexpr (RB > 2) if (sel & 4u) pair_j<R, AK, 2, PRED>(vr, vi, k, pm);
-    if constexpr (RB > 3) if (sel & 8u) pair_j<R, AK, 3, PRED>(vr, vi, k, pm);
+    if (hit(sel & 1u)) pair_j<R, AK, 0, PRED>(vr, vi, k, pm);
+    if constexpr (RB > 1) if (hit(sel & 2u)) pair_j<R, AK, 1, PRED>(vr, vi, k, pm);
+    if constexpr (RB > 2) if (hit(sel & 4u)) pair_j<R, AK, 2, PRED>(vr, vi, k, pm);
+    if constexpr (RB > 3) if (hit(sel & 8u)) pair_j<R, AK, 3, PRED>(vr, vi, k, pm);
 }
 
 template <int R, int RB, bool PRED>
 __device__ __forceinline__ void pair_kinds(uint32_t g, uint32_t sel, double *vr, double *vi,
                                            const double *k, uint32_t pm)
 {
-    if (g & TG_H) pair_sel<R, RB, AK_H, PRED>(sel, vr, vi, k, pm);
-    if (g & TG_X) pair_sel<R, RB, AK_X, PRED>(sel, vr, vi, k, pm);
-    if (g & TG_Y) pair_sel<R, RB, AK_Y, PRED>(sel, vr, vi, k, pm);
-    if (g & TG_RX) pair_sel<R, RB, AK_RX, PRED>(sel, vr, vi, k, pm);
-    if (g & TG_RY) pair_sel<R, RB, AK_RY, PRED>(sel, vr, vi, k, pm);
-    if (g & TG_U) pair_sel<R, RB, AK_U, PRED>(sel, vr, vi, k, pm);
+    if (hit(g & TG_H)) pair_sel<R, RB, AK_H, PRED>(sel, vr, vi, k, pm);
+    if (hit(g & TG_X)) pair_sel<R, RB, AK_X, PRED>(sel, vr, vi, k, pm);
+    if (hit(g & TG_Y)) pair_sel<R, RB, AK_Y, PRED>(sel, vr, vi, k, pm);
+    if (hit(g & TG_RX)) pair_sel<R, RB, AK_RX, PRED>(sel, vr, vi, k, pm);
+    if (hit(g & TG_RY)) pair_sel<R, RB, AK_RY, PRED>(sel, vr, vi, k, pm);
+    if (hit(g & TG_U)) pair_sel<R, RB, AK_U, PRED>(sel, vr, vi, k, pm);
 }
 
 // Diagonal update of one amplitude whose target bit is known at compile
@@ -126,10 +130,10 @@ template <int R, int RB, int AK, bool PRED>
 __device__ __forceinline__ void diag_sel(uint32_t sel, double *vr, double *vi, const double *k,
                                          uint32_t pm)
 {
-    if (sel & 1u) diag_j<R, AK, 0, PRED>(vr, vi, k, pm);
-    if constexpr (RB > 1) if (sel & 2u) diag_j<R, AK, 1, PRED>(vr, vi, k, pm);
-    if constexpr (RB > 2) if (sel & 4u) diag_j<R, AK, 2, PRED>(vr, vi, k, pm);
-    if constexpr (RB > 3) if (sel & 8u) diag_j<R, AK, 3, PRED>(vr, vi, k, pm);
+    if (hit(sel & 1u)) diag_j<R, AK, 0, PRED>(vr, vi, k, pm);
+    if constexpr (RB > 1) if (hit(sel & 2u)) diag_j<R, AK, 1, PRED>(vr, vi, k, pm);
+    if constexpr (RB > 2) if (hit(sel & 4u)) diag_j<R, AK, 2, PRED>(vr, vi, k, pm);
+    if constexpr (RB > 3) if (hit(sel & 8u)) diag_j<R, AK, 3, PRED>(vr, vi, k, pm);
 }
 
 // Diagonal target on a lane bit or outside the tile: `bit` per lane (or per
@@ -179,18 +183,18 @@ template <int R, int RB, bool PRED>
 __device__ __forceinline__ void diag_reg_kinds(uint32_t g, uint32_t sel, double *vr, double *vi,
                                                const double *k, uint32_t pm)
 {
-    if (g & TG_NEG) diag_sel<R, RB, AK_NEG, PRED>(sel, vr, vi, k, pm);
-    if (g & TG_PHASE) diag_sel<R, RB, AK_PHASE, PRED>(sel, vr, vi, k, pm);
-    if (g & TG_RZ) diag_sel<R, RB, AK_RZ, PRED>(sel, vr, vi, k, pm);
+    if (hit(g & TG_NEG)) diag_sel<R, RB, AK_NEG, PRED>(sel, vr, vi, k, pm);
+    if (hit(g & TG_PHASE)) diag_sel<R, RB, AK_PHASE, PRED>(sel, vr, vi, k, pm);
+    if (hit(g & TG_RZ)) diag_sel<R, RB, AK_RZ, PRED>(sel, vr, vi, k, pm);
 }
 
 template <int R, bool PRED>
 __device__ __forceinline__ void diag_lane_kinds(uint32_t g, double *vr, double *vi,
                                                 const double *k, bool bit, uint32_t pm)
 {
-    if (g & TG_NEG) diag_lane<R, AK_NEG, PRED>(vr, vi, k, bit, pm);
-    if (g & TG_PHASE) diag_lane<R, AK_PHASE, PRED>(vr, vi, k, bit, pm);
-    if (g & TG_RZ) diag_lane<R, AK_RZ, PRED>(vr, vi, k, bit, pm);
+    if (hit(g & TG_NEG)) diag_lane<R, AK_NEG, PRED>(vr, vi, k, bit, pm);
+    if (hit(g & TG_PHASE)) diag_lane<R, AK_PHASE, PRED>(vr, vi, k, bit, pm);
+    if (hit(g & TG_RZ)) diag_lane<R, AK_RZ, PRED>(vr, vi, k, bit, pm);
 }
 
 // Register budget: asking for 6 waves per SIMD makes the compiler fit the
@@ -209,10 +213,15 @@ k_tile(T *__restrict__ re, T *__restrict__ im, const TileDesc d, const TileOp *_
 
     // global bits of the qubits outside S for this tile
     const uint64_t base = insert_zeros((uint64_t)blockIdx.x, d.spos, K);
+    // thread part of the global index and of the local index (lane part)
     uint64_t gt = 0;
+    uint32_t lt = 0;
 #pragma unroll
     for (int i = 0; i < LOGW; i++)
-        if ((tid >> i) & 1u) gt |= 1ull << d.spos[i];
+        if ((tid >> i) & 1u) {
+            gt |= 1ull << d.spos[d.tin[i]];
+            lt |= 1u << d.tin[i];
+        }
 
     double vr[R], vi[R];
 #pragma unroll
@@ -221,9 +230,6 @@ k_tile(T *__restrict__ re, T *__restrict__ im, const TileDesc d, const TileOp *_
         vr[p] = (double)re[g];
         vi[p] = (double)im[g];
     }
-    // lane part of the local index: in the load layout the thread index holds
-    // local bits 0..LOGW-1 and register p holds bits LOGW..K-1
-    uint32_t lt = tid;
 
     const int nops = d.nops;
     for (int o = 0; o < nops; o++) {
@@ -235,7 +241,7 @@ k_tile(T *__restrict__ re, T *__restrict__ im, const TileDesc d, const TileOp *_
         asm volatile("" : "+s"(oi));
         const TileOp cur = ops[oi];
         const uint32_t g = cur.grp;
-        if (g & TG_LAYOUT) {
+        if (hit(g & TG_LAYOUT)) {
             uint32_t nlt = 0;
 #pragma unroll
             for (int i = 0; i < LOGW; i++) nlt |= ((tid >> i) & 1u) << cur.lay.newthr[i];
@@ -274,28 +280,33 @@ k_tile(T *__restrict__ re, T *__restrict__ im, const TileDesc d, const TileOp *_
             // controls on lane bits: a per-lane predicate inside the predicated paths
             const bool lane_ok = (lt & cur.mlo) == cur.mlo && !(lt & cur.mlz);
             const double *k = cur.c;
-            if (g & TG_PAIR) pair_kinds<R, RB, false>(g, sel, vr, vi, k, 0);
+            if (hit(g & TG_PAIR)) pair_kinds<R, RB, false>(g, sel, vr, vi, k, 0);
             // predicated ops: the lane part of the controls is one divergent
             // branch around the op, the register part a uniform test per register
-            if (g & TG_PAIRP)
+            if (hit(g & TG_PAIRP))
                 if (lane_ok) pair_kinds<R, RB, true>(g, sel, vr, vi, k, pm);
-            if (g & TG_DREG) diag_reg_kinds<R, RB, false>(g, sel, vr, vi, k, 0);
-            if (g & TG_DREGP)
+            if (hit(g & TG_DREG)) diag_reg_kinds<R, RB, false>(g, sel, vr, vi, k, 0);
+            if (hit(g & TG_DREGP))
                 if (lane_ok) diag_reg_kinds<R, RB, true>(g, sel, vr, vi, k, pm);
-            if (g & (TG_DLANE | TG_DLANEP)) {
+            if (hit(g & (TG_DLANE | TG_DLANEP))) {
                 // target on a lane bit (dsel 1) or outside the tile (dsel 2)
                 const bool bit = cur.dsel == 2 ? (base & cur.ebit) != 0
                                                : ((lt >> cur.dpos) & 1u) != 0;
-                if (g & TG_DLANE) diag_lane_kinds<R, false>(g, vr, vi, k, bit, 0);
-                if (g & TG_DLANEP)
+                if (hit(g & TG_DLANE)) diag_lane_kinds<R, false>(g, vr, vi, k, bit, 0);
+                if (hit(g & TG_DLANEP))
                     if (lane_ok) diag_lane_kinds<R, true>(g, vr, vi, k, bit, pm);
             }
         }
     }
 
+    // store layout: the lane part is where the last layout change left it
+    uint64_t gs = 0;
+#pragma unroll
+    for (int b = 0; b < K; b++)
+        if ((lt >> b) & 1u) gs |= 1ull << d.spos[b];
 #pragma unroll
     for (int p = 0; p < R; p++) {
-        const uint64_t g = base | gt | d.gr[p];
+        const uint64_t g = base | gs | d.gs[p];
         re[g] = (T)vr[p];
         im[g] = (T)vi[p];
     }
