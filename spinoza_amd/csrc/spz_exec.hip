@@ -14,6 +14,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <mutex>
 #include <vector>
 
 #include "spz_internal.h"
@@ -583,9 +584,50 @@ struct Planner {
     }
 };
 
-// Device program buffer (grown as needed, reused across calls).
-TileOp *g_dprog = nullptr;
-size_t g_dprog_cap = 0;
+// Device programs: a ring of two slots, each a pinned host staging buffer, a
+// device buffer (both grown as needed) and an event recorded after the last
+// sweep that reads it.  A call plans on the host while the GPU may still run
+// the previous call's sweeps, and waits only before it overwrites a slot
+// whose sweeps have not finished: execute calls return without a device
+// synchronisation, in stream order like every other entry point.
+struct ProgSlot {
+    TileOp *host = nullptr;
+    TileOp *dev = nullptr;
+    size_t cap = 0;
+    hipEvent_t done = nullptr;
+    bool pending = false;
+};
+ProgSlot g_slots[2];
+int g_slot_next = 0;
+std::mutex g_slot_mu;
+
+int upload_program(const std::vector<TileOp> &prog, ProgSlot **out)
+{
+    ProgSlot &sl = g_slots[g_slot_next];
+    g_slot_next ^= 1;
+    if (sl.pending) {
+        SPZ_HIP(hipEventSynchronize(sl.done));
+        sl.pending = false;
+    }
+    if (!sl.done) SPZ_HIP(hipEventCreateWithFlags(&sl.done, hipEventDisableTiming));
+    if (prog.size() > sl.cap) {
+        if (sl.dev) SPZ_HIP(hipFree(sl.dev));
+        if (sl.host) SPZ_HIP(hipHostFree(sl.host));
+        sl.dev = nullptr;
+        sl.host = nullptr;
+        sl.cap = 0;
+        const size_t cap = std::max(prog.size(), (size_t)4096);
+        if (hipMalloc(&sl.dev, cap * sizeof(TileOp)) != hipSuccess ||
+            hipHostMalloc(&sl.host, cap * sizeof(TileOp), hipHostMallocDefault) != hipSuccess)
+            return fail(SPZ_ERR_OOM, "tile program allocation failed");
+        sl.cap = cap;
+    }
+    memcpy(sl.host, prog.data(), prog.size() * sizeof(TileOp));
+    SPZ_HIP(hipMemcpyAsync(sl.dev, sl.host, prog.size() * sizeof(TileOp), hipMemcpyHostToDevice,
+                           rt().stream));
+    *out = &sl;
+    return SPZ_OK;
+}
 
 int run_fused(spz_state_s *s, const std::vector<SG> &run, int K, bool strict)
 {
@@ -599,29 +641,30 @@ int run_fused(spz_state_s *s, const std::vector<SG> &run, int K, bool strict)
     pl.strict = strict;
     pl.plan(run);
     if (pl.bad) return fail(SPZ_ERR_ARG, "tile planner produced a gate outside the register layout");
-    if (!pl.prog.empty()) {
-        SPZ_HIP(hipStreamSynchronize(rt().stream));  // the previous program may still be in use
-        if (pl.prog.size() > g_dprog_cap) {
-            if (g_dprog) SPZ_HIP(hipFree(g_dprog));
-            g_dprog = nullptr;
-            size_t cap = std::max(pl.prog.size(), (size_t)4096);
-            if (hipMalloc(&g_dprog, cap * sizeof(TileOp)) != hipSuccess)
-                return fail(SPZ_ERR_OOM, "tile program allocation failed");
-            g_dprog_cap = cap;
+    std::lock_guard<std::mutex> lk(g_slot_mu);
+    ProgSlot *slot = nullptr;
+    if (!pl.prog.empty()) SPZ_TRY(upload_program(pl.prog, &slot));
+    const TileOp *d_prog = slot ? slot->dev : nullptr;
+    // on every exit after the upload: mark the slot busy until the stream
+    // passes this point (an error return included, with the copy in flight)
+    struct Release {
+        ProgSlot *sl;
+        ~Release()
+        {
+            if (!sl) return;
+            if (hipEventRecord(sl->done, rt().stream) == hipSuccess) sl->pending = true;
+            else (void)hipStreamSynchronize(rt().stream);
         }
-        SPZ_HIP(hipMemcpyAsync(g_dprog, pl.prog.data(), pl.prog.size() * sizeof(TileOp),
-                               hipMemcpyHostToDevice, rt().stream));
-    }
+    } release{slot};
     for (const Sweep &sw : pl.sweeps) {
         if (sw.tile) {
-            SPZ_TRY(launch_tile(s, K, sw.desc, g_dprog + sw.op_begin));
+            SPZ_TRY(launch_tile(s, K, sw.desc, d_prog + sw.op_begin));
         } else {
             const SG &g = run[sw.gate];
             if (g.diag) SPZ_TRY(launch_diag(s, g.k, g.target, g.ones, g.zeros));
             else SPZ_TRY(launch_pair(s, g.k, g.target, g.ones, g.zeros));
         }
     }
-    if (!pl.prog.empty()) SPZ_HIP(hipStreamSynchronize(rt().stream));  // program buffer reuse
     return SPZ_OK;
 }
 
