@@ -282,6 +282,93 @@ struct Planner {
         prog.push_back(op);
     }
 
+    // Dependency graph of a sweep's gates: a -> b (a before b) when they share
+    // a qubit on which at least one acts non-diagonally.
+    void deps_of(const std::vector<SG> &run, const std::vector<int> &order,
+                 std::vector<std::vector<int>> &succ, std::vector<int> &indeg) const
+    {
+        const int m = (int)order.size();
+        succ.assign(m, {});
+        indeg.assign(m, 0);
+        for (int a = 0; a < m; a++) {
+            const SG &x = run[order[a]];
+            for (int b = a + 1; b < m; b++) {
+                const SG &y = run[order[b]];
+                if ((x.nd & (y.nd | y.dm)) || (x.dm & y.nd)) {
+                    succ[a].push_back(b);
+                    indeg[b]++;
+                }
+            }
+        }
+    }
+
+    // Gates that run, in dependency order, under register bits rs before a
+    // pair gate on another bit blocks every remaining one.
+    int runs_under(const std::vector<SG> &run, const std::vector<int> &order, const int *lbit,
+                   const std::vector<std::vector<int>> &succ, std::vector<int> indeg,
+                   std::vector<char> done, uint32_t rs) const
+    {
+        const int m = (int)order.size();
+        int cnt = 0;
+        bool progress = true;
+        while (progress) {
+            progress = false;
+            for (int a = 0; a < m; a++) {
+                if (done[a] || indeg[a]) continue;
+                const SG &g = run[order[a]];
+                if (!g.diag && !((rs >> lbit[g.target]) & 1)) continue;
+                done[a] = 1;
+                cnt++;
+                for (int b : succ[a]) indeg[b]--;
+                progress = true;
+            }
+        }
+        return cnt;
+    }
+
+    // The RB register bits (from `lo` up) to switch to: every RB-subset of
+    // the (up to 8) best-scored bits is tried, and the one under which the
+    // most gates run wins (ties: the higher score, then the earlier subset).
+    std::vector<int> best_set(const std::vector<SG> &run, const std::vector<int> &order,
+                              const int *lbit, const std::vector<std::vector<int>> &succ,
+                              const std::vector<int> &indeg, const std::vector<char> &done,
+                              const std::vector<long> &score, int lo) const
+    {
+        int left = 0;
+        for (char d : done) left += !d;
+        std::vector<int> bits;
+        for (int b = lo; b < K; b++) bits.push_back(b);
+        std::stable_sort(bits.begin(), bits.end(),
+                         [&](int x, int y) { return score[x] > score[y]; });
+        const int T = std::min((int)bits.size(), std::max(RB, 8));
+        uint32_t best = 0;
+        int best_cnt = -1;
+        long best_sc = -1;
+        for (uint32_t sub = 0; sub < (1u << T); sub++) {
+            if (__builtin_popcount(sub) != RB) continue;
+            uint32_t rs = 0;
+            long sc = 0;
+            for (int i = 0; i < T; i++)
+                if ((sub >> i) & 1) {
+                    rs |= 1u << bits[i];
+                    sc += score[bits[i]];
+                }
+            int cnt = runs_under(run, order, lbit, succ, indeg, done, rs);
+            // a set that finishes the sweep and keeps the low bits on lanes
+            // saves the change back before the store
+            if (cnt == left && !(rs & ((1u << klo) - 1))) cnt++;
+            if (cnt > best_cnt || (cnt == best_cnt && sc > best_sc)) {
+                best = rs;
+                best_cnt = cnt;
+                best_sc = sc;
+            }
+        }
+        std::vector<int> out;
+        for (int b = 0; b < K; b++)
+            if ((best >> b) & 1) out.push_back(b);
+        return out;
+    }
+
     // Sequence the sweep's gates: the scheduled order itself (strict), or,
     // with reordering allowed, a topological order of the dependency graph
     // that applies every gate the current register layout allows before
@@ -302,27 +389,16 @@ struct Planner {
                 if (RS[j] == b) return true;
             return false;
         };
-        if (!strict) {
+        if (!strict && K - klo >= RB) {
+            std::vector<std::vector<int>> succ0;
+            std::vector<int> indeg0;
+            deps_of(run, order, succ0, indeg0);
             std::vector<long> score(K, 0);
-            {
-                std::vector<int> deps(m, 0);
-                for (int a = 0; a < m; a++) {
-                    const SG &x = run[order[a]];
-                    for (int b = a + 1; b < m; b++) {
-                        const SG &y = run[order[b]];
-                        if ((x.nd & (y.nd | y.dm)) || (x.dm & y.nd)) deps[b]++;
-                    }
-                }
-                for (int a = 0; a < m; a++) {
-                    const SG &g = run[order[a]];
-                    if (!g.diag) score[lbit[g.target]] += deps[a] == 0 ? 1000 : 1;
-                }
+            for (int a = 0; a < m; a++) {
+                const SG &g = run[order[a]];
+                if (!g.diag) score[lbit[g.target]] += indeg0[a] == 0 ? 1000 : 1;
             }
-            std::vector<int> bits;
-            for (int b = klo; b < K; b++) bits.push_back(b);
-            std::stable_sort(bits.begin(), bits.end(),
-                             [&](int x, int y) { return score[x] > score[y]; });
-            if ((int)bits.size() >= RB) RS.assign(bits.begin(), bits.begin() + RB);
+            RS = best_set(run, order, lbit, succ0, indeg0, std::vector<char>(m, 0), score, klo);
         }
         rs_in = RS;
         if (strict) {
@@ -361,18 +437,9 @@ struct Planner {
         }
         // dependency graph: i -> j (i before j) when they share a qubit on
         // which at least one acts non-diagonally
-        std::vector<std::vector<int>> succ(m);
-        std::vector<int> indeg(m, 0);
-        for (int a = 0; a < m; a++) {
-            const SG &x = run[order[a]];
-            for (int b = a + 1; b < m; b++) {
-                const SG &y = run[order[b]];
-                if ((x.nd & (y.nd | y.dm)) || (x.dm & y.nd)) {
-                    succ[a].push_back(b);
-                    indeg[b]++;
-                }
-            }
-        }
+        std::vector<std::vector<int>> succ;
+        std::vector<int> indeg;
+        deps_of(run, order, succ, indeg);
         std::vector<char> done(m, 0);
         int left = m;
         while (left > 0) {
@@ -392,7 +459,9 @@ struct Planner {
             }
             if (left == 0) break;
             // next layout: score target bits of the remaining non-diagonal
-            // gates, ready ones first
+            // gates, ready ones first; among the RB-subsets of the best
+            // scored bits, take the one under which the most gates run
+            // before the next change (ties: the higher score)
             std::vector<long> score(K, 0);
             for (int a = 0; a < m; a++) {
                 if (done[a]) continue;
@@ -400,11 +469,7 @@ struct Planner {
                 if (g.diag) continue;
                 score[lbit[g.target]] += indeg[a] == 0 ? 1000 : 1;
             }
-            std::vector<int> bits(K);
-            for (int b = 0; b < K; b++) bits[b] = b;
-            std::stable_sort(bits.begin(), bits.end(),
-                             [&](int x, int y) { return score[x] > score[y]; });
-            std::vector<int> nx(bits.begin(), bits.begin() + RB);
+            std::vector<int> nx = best_set(run, order, lbit, succ, indeg, done, score, 0);
             // keep bits that stay in place in their old slots
             std::vector<int> slotted(RB, -1);
             for (int j = 0; j < RB; j++)
