@@ -197,6 +197,14 @@ __device__ __forceinline__ void diag_lane_kinds(uint32_t g, double *vr, double *
     if (hit(g & TG_RZ)) diag_lane<R, AK_RZ, PRED>(vr, vi, k, bit, pm);
 }
 
+// Controls on lane bits, evaluated inside each predicated path (the opaque
+// copy keeps the compiler from hoisting the test into every op).
+__device__ __forceinline__ bool lane_pass(uint32_t lt, uint32_t mlo, uint32_t mlz)
+{
+    asm volatile("" : "+v"(lt));
+    return (lt & mlo) == mlo && !(lt & mlz);
+}
+
 // Register budget: asking for 6 waves per SIMD makes the compiler fit the
 // kernel in ~102 VGPRs without spills (4 resident waves per SIMD instead of 3
 // at its unconstrained 138); the extra wave lets one workgroup's loads and
@@ -232,14 +240,25 @@ k_tile(T *__restrict__ re, T *__restrict__ im, const TileDesc d, const TileOp *_
     }
 
     const int nops = d.nops;
+    uint64_t opa = reinterpret_cast<uint64_t>(ops);
     for (int o = 0; o < nops; o++) {
         // by value: the op's fields come into scalar registers, so selects
-        // between its coefficients are value selects, never per-lane loads
-        // (the opaque index keeps every field at an immediate offset of one
-        // scalar address instead of strength-reduced bases per field)
-        int oi = o;
-        asm volatile("" : "+s"(oi));
-        const TileOp cur = ops[oi];
+        // between its coefficients are value selects, never per-lane loads.
+        // The opaque address read through the constant address space gives
+        // scalar loads at immediate offsets of one base that advances by one
+        // add per op (not strength-reduced bases per field).
+        uint64_t pa = opa;
+        asm volatile("" : "+s"(pa));
+        TileOp cur;
+        {
+            const __attribute__((address_space(4))) uint64_t *q =
+                reinterpret_cast<const __attribute__((address_space(4))) uint64_t *>(pa);
+            uint64_t w[sizeof(TileOp) / 8];
+#pragma unroll
+            for (int i = 0; i < (int)(sizeof(TileOp) / 8); i++) w[i] = q[i];
+            __builtin_memcpy(&cur, w, sizeof(TileOp));
+        }
+        opa += sizeof(TileOp);
         const uint32_t g = cur.grp;
         if (hit(g & TG_LAYOUT)) {
             uint32_t nlt = 0;
@@ -277,24 +296,22 @@ k_tile(T *__restrict__ re, T *__restrict__ im, const TileDesc d, const TileOp *_
         // (a control outside the tile that is not met skips the op)
         if ((base & cur.em) == cur.eo) {
             const uint32_t sel = cur.sel, pm = cur.pm;
-            // controls on lane bits: a per-lane predicate inside the predicated paths
-            const bool lane_ok = (lt & cur.mlo) == cur.mlo && !(lt & cur.mlz);
             const double *k = cur.c;
             if (hit(g & TG_PAIR)) pair_kinds<R, RB, false>(g, sel, vr, vi, k, 0);
             // predicated ops: the lane part of the controls is one divergent
             // branch around the op, the register part a uniform test per register
             if (hit(g & TG_PAIRP))
-                if (lane_ok) pair_kinds<R, RB, true>(g, sel, vr, vi, k, pm);
+                if (lane_pass(lt, cur.mlo, cur.mlz)) pair_kinds<R, RB, true>(g, sel, vr, vi, k, pm);
             if (hit(g & TG_DREG)) diag_reg_kinds<R, RB, false>(g, sel, vr, vi, k, 0);
             if (hit(g & TG_DREGP))
-                if (lane_ok) diag_reg_kinds<R, RB, true>(g, sel, vr, vi, k, pm);
+                if (lane_pass(lt, cur.mlo, cur.mlz)) diag_reg_kinds<R, RB, true>(g, sel, vr, vi, k, pm);
             if (hit(g & (TG_DLANE | TG_DLANEP))) {
                 // target on a lane bit (dsel 1) or outside the tile (dsel 2)
                 const bool bit = cur.dsel == 2 ? (base & cur.ebit) != 0
                                                : ((lt >> cur.dpos) & 1u) != 0;
                 if (hit(g & TG_DLANE)) diag_lane_kinds<R, false>(g, vr, vi, k, bit, 0);
                 if (hit(g & TG_DLANEP))
-                    if (lane_ok) diag_lane_kinds<R, true>(g, vr, vi, k, bit, pm);
+                    if (lane_pass(lt, cur.mlo, cur.mlz)) diag_lane_kinds<R, true>(g, vr, vi, k, bit, pm);
             }
         }
     }
