@@ -239,26 +239,23 @@ k_tile(T *__restrict__ re, T *__restrict__ im, const TileDesc d, const TileOp *_
         vi[p] = (double)im[g];
     }
 
-    const int nops = d.nops;
-    uint64_t opa = reinterpret_cast<uint64_t>(ops);
-    for (int o = 0; o < nops; o++) {
+    const uint64_t ope = reinterpret_cast<uint64_t>(ops + d.nops);
+    for (uint64_t opa = reinterpret_cast<uint64_t>(ops); opa != ope; opa += sizeof(TileOp)) {
         // by value: the op's fields come into scalar registers, so selects
         // between its coefficients are value selects, never per-lane loads.
         // The opaque address read through the constant address space gives
         // scalar loads at immediate offsets of one base that advances by one
         // add per op (not strength-reduced bases per field).
-        uint64_t pa = opa;
-        asm volatile("" : "+s"(pa));
+        asm volatile("" : "+s"(opa));
         TileOp cur;
         {
             const __attribute__((address_space(4))) uint64_t *q =
-                reinterpret_cast<const __attribute__((address_space(4))) uint64_t *>(pa);
+                reinterpret_cast<const __attribute__((address_space(4))) uint64_t *>(opa);
             uint64_t w[sizeof(TileOp) / 8];
 #pragma unroll
             for (int i = 0; i < (int)(sizeof(TileOp) / 8); i++) w[i] = q[i];
             __builtin_memcpy(&cur, w, sizeof(TileOp));
         }
-        opa += sizeof(TileOp);
         const uint32_t g = cur.grp;
         if (hit(g & TG_LAYOUT)) {
             uint32_t nlt = 0;
