@@ -206,7 +206,7 @@ __device__ __forceinline__ bool lane_pass(uint32_t lt, uint32_t mlo, uint32_t ml
 }
 
 // Register budget: asking for 6 waves per SIMD makes the compiler fit the
-// kernel in ~102 VGPRs without spills (4 resident waves per SIMD instead of 3
+// kernel in ~108 VGPRs without spills (4 resident waves per SIMD instead of 3
 // at its unconstrained 138); the extra wave lets one workgroup's loads and
 // stores overlap the others' gate arithmetic.
 template <typename T, int K, int LOGW>
