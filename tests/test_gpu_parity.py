@@ -419,3 +419,29 @@ def test_controlled_gates_in_tiles_vs_oracle(spz, orc):
         s = to_gpu(spz, base)
         run_ops(s, ops, mode=mode)
         assert_match(s, ref, exact=(mode == N.EXEC_STRICT), what=f"controlled tiles mode={mode}")
+
+
+@pytest.mark.gpu
+def test_back_to_back_executes_without_sync(spz, orc):
+    """Execute calls return without a device synchronisation and the tile
+    programs rotate through two staging slots: five different circuits queued
+    back to back on five states (the third call reuses the first call's slot
+    while earlier sweeps may still run) must each match the oracle."""
+    print("case back-to-back executes", flush=True)
+    from spinoza_amd import _native as N
+    from spinoza_amd.executor import run_ops
+    from spinoza_amd.workloads import random_circuit
+    n = 16
+    cases = []
+    for k in range(5):
+        ops = random_circuit(n, 6 + 3 * k, 100 + k)
+        base = orc.random_state(20 + k, n)
+        ref = base.copy()
+        orc.execute(ref, ops)
+        cases.append((ops, base, ref))
+    states = [to_gpu(spz, base) for _, base, _ in cases]
+    N.check(N.lib().spz_sync())
+    for s, (ops, _, _) in zip(states, cases):
+        run_ops(s, ops, mode=N.EXEC_STRICT)
+    for k, (s, (_, _, ref)) in enumerate(zip(states, cases)):
+        assert_match(s, ref, exact=True, what=f"back-to-back circuit {k}")
