@@ -84,12 +84,16 @@ template <int R, int RB, bool PRED>
 __device__ __forceinline__ void pair_kinds(uint32_t g, uint32_t sel, double *vr, double *vi,
                                            const double *k, uint32_t pm)
 {
-    if (hit(g & TG_H)) pair_sel<R, RB, AK_H, PRED>(sel, vr, vi, k, pm);
-    if (hit(g & TG_X)) pair_sel<R, RB, AK_X, PRED>(sel, vr, vi, k, pm);
-    if (hit(g & TG_Y)) pair_sel<R, RB, AK_Y, PRED>(sel, vr, vi, k, pm);
-    if (hit(g & TG_RX)) pair_sel<R, RB, AK_RX, PRED>(sel, vr, vi, k, pm);
-    if (hit(g & TG_RY)) pair_sel<R, RB, AK_RY, PRED>(sel, vr, vi, k, pm);
-    if (hit(g & TG_U)) pair_sel<R, RB, AK_U, PRED>(sel, vr, vi, k, pm);
+    if (hit(g & (TG_H | TG_X | TG_Y))) {
+        if (hit(g & TG_H)) pair_sel<R, RB, AK_H, PRED>(sel, vr, vi, k, pm);
+        if (hit(g & TG_X)) pair_sel<R, RB, AK_X, PRED>(sel, vr, vi, k, pm);
+        if (hit(g & TG_Y)) pair_sel<R, RB, AK_Y, PRED>(sel, vr, vi, k, pm);
+    }
+    if (hit(g & (TG_RX | TG_RY | TG_U))) {
+        if (hit(g & TG_RX)) pair_sel<R, RB, AK_RX, PRED>(sel, vr, vi, k, pm);
+        if (hit(g & TG_RY)) pair_sel<R, RB, AK_RY, PRED>(sel, vr, vi, k, pm);
+        if (hit(g & TG_U)) pair_sel<R, RB, AK_U, PRED>(sel, vr, vi, k, pm);
+    }
 }
 
 // Diagonal update of one amplitude whose target bit is known at compile
@@ -294,21 +298,27 @@ k_tile(T *__restrict__ re, T *__restrict__ im, const TileDesc d, const TileOp *_
         if ((base & cur.em) == cur.eo) {
             const uint32_t sel = cur.sel, pm = cur.pm;
             const double *k = cur.c;
-            if (hit(g & TG_PAIR)) pair_kinds<R, RB, false>(g, sel, vr, vi, k, 0);
-            // predicated ops: the lane part of the controls is one divergent
-            // branch around the op, the register part a uniform test per register
-            if (hit(g & TG_PAIRP))
-                if (lane_pass(lt, cur.mlo, cur.mlz)) pair_kinds<R, RB, true>(g, sel, vr, vi, k, pm);
-            if (hit(g & TG_DREG)) diag_reg_kinds<R, RB, false>(g, sel, vr, vi, k, 0);
-            if (hit(g & TG_DREGP))
-                if (lane_pass(lt, cur.mlo, cur.mlz)) diag_reg_kinds<R, RB, true>(g, sel, vr, vi, k, pm);
-            if (hit(g & (TG_DLANE | TG_DLANEP))) {
-                // target on a lane bit (dsel 1) or outside the tile (dsel 2)
-                const bool bit = cur.dsel == 2 ? (base & cur.ebit) != 0
-                                               : ((lt >> cur.dpos) & 1u) != 0;
-                if (hit(g & TG_DLANE)) diag_lane_kinds<R, false>(g, vr, vi, k, bit, 0);
-                if (hit(g & TG_DLANEP))
-                    if (lane_pass(lt, cur.mlo, cur.mlz)) diag_lane_kinds<R, true>(g, vr, vi, k, bit, pm);
+            // (two group tests, so an op skips the other family with one test)
+            if (hit(g & (TG_PAIR | TG_PAIRP))) {
+                if (hit(g & TG_PAIR)) pair_kinds<R, RB, false>(g, sel, vr, vi, k, 0);
+                // predicated ops: the lane part of the controls is one divergent
+                // branch around the op, the register part a uniform test per register
+                if (hit(g & TG_PAIRP))
+                    if (lane_pass(lt, cur.mlo, cur.mlz)) pair_kinds<R, RB, true>(g, sel, vr, vi, k, pm);
+            }
+            if (hit(g & (TG_DREG | TG_DREGP | TG_DLANE | TG_DLANEP))) {
+                if (hit(g & TG_DREG)) diag_reg_kinds<R, RB, false>(g, sel, vr, vi, k, 0);
+                if (hit(g & TG_DREGP))
+                    if (lane_pass(lt, cur.mlo, cur.mlz)) diag_reg_kinds<R, RB, true>(g, sel, vr, vi, k, pm);
+                if (hit(g & (TG_DLANE | TG_DLANEP))) {
+                    // target on a lane bit (dsel 1) or outside the tile (dsel 2)
+                    const bool bit = cur.dsel == 2 ? (base & cur.ebit) != 0
+                                                   : ((lt >> cur.dpos) & 1u) != 0;
+                    if (hit(g & TG_DLANE)) diag_lane_kinds<R, false>(g, vr, vi, k, bit, 0);
+                    if (hit(g & TG_DLANEP))
+                        if (lane_pass(lt, cur.mlo, cur.mlz))
+                            diag_lane_kinds<R, true>(g, vr, vi, k, bit, pm);
+                }
             }
         }
     }
