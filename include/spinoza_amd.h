@@ -15,8 +15,11 @@
  *     separate arrays (reals, imags), FP64 or FP32 -- the SoA layout of
  *     State { reals, imags, n } (core.rs:18-51).
  *   - All work on states is ordered on the library's stream (spz_stream()).
- *     Calls on one handle must not overlap from several threads (the
- *     reference takes &mut State); calls on different handles may.
+ *     One library-wide lock serialises the entry points that use shared
+ *     scratch (reductions, tile programs, block fragments, adjoint and batch
+ *     buffers), so calls on different handles may come from several threads.
+ *     Calls on one handle still must not overlap (the reference takes
+ *     &mut State): their order would be undefined.
  *   - Qubit q is bit q of the amplitude index (qubit 0 = least significant),
  *     as in the reference.
  */
@@ -156,8 +159,12 @@ typedef struct spz_tracker {
 enum spz_exec_mode {
     SPZ_EXEC_AUTO = 0,      /* fused tile sweeps where they pay off         */
     SPZ_EXEC_PLAIN = 1,     /* one kernel launch per gate, in program order */
-    SPZ_EXEC_STRICT = 2,    /* fused sweeps that keep program order exactly
-                               (results bit-identical to PLAIN)             */
+    SPZ_EXEC_STRICT = 2,    /* fused sweeps that keep program order exactly;
+                               FP64 results are bit-identical to PLAIN.  With
+                               FP32 storage a sweep keeps FP64 values between
+                               its gates and rounds once at the store, where
+                               PLAIN rounds after every gate: the two agree
+                               within FP32 rounding, not bit for bit         */
     SPZ_EXEC_DENSE5 = 3     /* runs of gates on <= 5 qubits fused into dense
                                32x32 blocks applied with MFMA (FP64 or FP32
                                to match storage); needs n >= 12             */
@@ -176,6 +183,19 @@ typedef struct spz_exec_opts {
  * not a fresh measurement).  opts may be NULL (SPZ_EXEC_AUTO). */
 int spz_execute(spz_state s, const spz_op *ops, uint64_t n_ops, spz_tracker *tracker,
                 const spz_exec_opts *opts, uint8_t *outcomes);
+
+/* ---- OpenQASM 2.0 loader: openqasm::load / loads (openqasm.rs:13-171) ---
+ * Parse the program text src[0..len) into the ops that the reference's
+ * build_circuit adds (qreg declarations; h x y z rx ry rz u on each listed
+ * qubit; cp(theta) and cx with the control first; angles may use pi).
+ * *n_qubits receives the total register width (registers in declaration
+ * order), *n_ops the op count.  With cap = 0 (ops may be NULL) only the
+ * counts are returned; otherwise ops[0..cap) must hold them all.  Errors
+ * (unsupported statements, unknown registers, indices out of range) return
+ * SPZ_ERR_ARG with the line in spz_last_error().  Host only: no device is
+ * needed. */
+int spz_qasm_parse(const char *src, uint64_t len, spz_op *ops, uint64_t cap, uint64_t *n_ops,
+                   int *n_qubits);
 
 /* ---- measurement and expectation (measurement.rs, core.rs) -------------- */
 /* sum |psi|^2 with the fixed compensated FP64 tree. */
@@ -253,8 +273,12 @@ typedef struct spz_batch_stats {
 
 /* Energy and adjoint gradient (as spz_adjoint_grad, no checkpoints) of every
  * circuit: energies[i], grads[grad_offset + k] for op k of circuit i.  Ops
- * may be single-target gates H..U with up to 7 controls.  opts and stats may
- * be NULL. */
+ * may be single-target gates H..U with up to 7 controls (1- or 0-controls,
+ * none equal to the target, no qubit both).  Every op of every circuit is
+ * checked before any work starts, by the same rule whichever path (the
+ * persistent kernel or the single-circuit one) a circuit would take: one bad
+ * op fails the whole call with SPZ_ERR_GATE / SPZ_ERR_QUBIT.  opts and stats
+ * may be NULL. */
 int spz_batch_adjoint(const spz_batch_circuit *circuits, uint64_t count, const spz_op *ops,
                       double hx, double *energies, double *grads,
                       const spz_batch_opts *opts, spz_batch_stats *stats);

@@ -288,24 +288,36 @@ static inline void orc_pair(double *re, double *im, uint64_t s0, uint64_t s1,
     }
 }
 
-/* Iterate every pair (s0, s1) with bit `target` clear in s0, every bit of
- * `ones` set and every bit of `zeros` clear.  The reference's loops
- * (chunked for plain gates, gates.rs:336-344; insert-bit maps for one
- * control, gates.rs:386-388; mask scans for u_c_apply and the multi-control
- * arms, gates.rs:1354-1362) all visit exactly this set of disjoint pairs, so
- * the visiting order does not change the result. */
+/* Visit every pair (s0, s1 = s0 | 2^target) with every bit of `ones` set and
+ * every bit of `zeros` clear in s0, in the reference's decomposition: pair
+ * index p in [0, N / 2^(1 + #controls)) is spread into s0 by inserting a zero
+ * bit at each fixed position (target and controls, ascending), then the
+ * 1-controls are set.  Without controls this is the chunked loop of
+ * gates.rs:336-344 (chunks of 2^(t+1), low half paired with high half:
+ * s0 = chunk * 2^(t+1) + i, contiguous runs of 2^t for each thread); with one
+ * control it is the insert-bit map of gates.rs:386-388.  The reference's mask
+ * scans (u_c_apply, gates.rs:1354-1362, and the multi-control arms) visit the
+ * same disjoint pairs, so the order does not change the result.  Threads
+ * split p into contiguous ranges at n >= LOW_QUBIT_THRESHOLD, as rayon does
+ * above gates.rs:14's threshold. */
 static void orc_for_pairs(double *re, double *im, int n, int target,
                           uint64_t ones, uint64_t zeros, const orc_coef *k)
 {
-    uint64_t N = (uint64_t)1 << n;
     uint64_t tb = (uint64_t)1 << target;
-    int64_t i;
+    uint64_t fixed = tb | ones | zeros;
+    int pos[64], np = 0;
+    for (int q = 0; q < n; q++)
+        if ((fixed >> q) & 1) pos[np++] = q;
+    uint64_t M = (uint64_t)1 << (n - np);
+    int64_t p;
 #pragma omp parallel for ORC_PAR(n) schedule(static)
-    for (i = 0; i < (int64_t)N; i++) {
-        uint64_t s0 = (uint64_t)i;
-        if (s0 & tb) continue;
-        if ((s0 & ones) != ones) continue;
-        if (s0 & zeros) continue;
+    for (p = 0; p < (int64_t)M; p++) {
+        uint64_t s0 = (uint64_t)p;
+        for (int i = 0; i < np; i++) {
+            int q = pos[i];
+            s0 = ((s0 >> q) << (q + 1)) | (s0 & (((uint64_t)1 << q) - 1));
+        }
+        s0 |= ones;
         orc_pair(re, im, s0, s0 | tb, k);
     }
 }

@@ -4,6 +4,7 @@
 There is no CPU fallback: if the library is missing, importing the package
 raises, and if no GPU is visible every call fails with SpinozaError.
 """
+import atexit
 import ctypes
 import os
 
@@ -129,12 +130,31 @@ SIGNATURES = {
                                 ctypes.POINTER(_i)]),
     "spz_plan_stats": (_i, [_i, _i, ctypes.POINTER(Op_t), _u64, ctypes.POINTER(ExecOpts_t),
                            ctypes.POINTER(PlanInfo_t)]),
+    "spz_qasm_parse": (_i, [ctypes.c_char_p, _u64, ctypes.POINTER(Op_t), _u64, _u64p,
+                            ctypes.POINTER(_i)]),
     "spz_profile_enable": (_i, [_i]),
     "spz_profile_reset": (_i, []),
     "spz_profile_get": (_i, [_i, _u64p, _dp, _dp]),
 }
 
 _lib = None
+_exiting = False
+
+
+def _mark_exit():
+    global _exiting
+    _exiting = True
+
+
+# States still alive when the interpreter shuts down (held by a module global
+# or a stored traceback) are not freed one by one: the HIP runtime may already
+# be tearing down while Python finalises objects, and the process's device
+# memory goes with the process anyway.
+atexit.register(_mark_exit)
+
+
+def exiting():
+    return _exiting
 
 
 def load():

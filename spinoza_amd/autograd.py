@@ -75,6 +75,20 @@ class IsingEnergy:
     tensor whose gradient w.r.t. `thetas` is the adjoint gradient."""
 
     _fn = None
+    # One working state per (qubits, precision), reset to |0...0> on every
+    # call: an optimiser loop then allocates (and frees, which synchronises
+    # the device) nothing after its first step.
+    _states = {}
+
+    @classmethod
+    def _state(cls, n, precision):
+        s = cls._states.get((n, precision))
+        if s is None:
+            s = State(n, precision)
+            cls._states[(n, precision)] = s
+        else:
+            s.set_basis(0)
+        return s
 
     @classmethod
     def apply(cls, thetas, builder, n, hx=1.0, ckpt_every=0, precision="f64"):
@@ -86,7 +100,7 @@ class IsingEnergy:
                 def forward(ctx, th, builder_, n_, hx_, ck_, prec_):
                     vals = th.detach().to("cpu", torch.float64).tolist()
                     ops, param_ops = builder_(vals)
-                    s = State(n_, prec_)
+                    s = cls._state(n_, prec_)
                     e, g, _ = adjoint_gradient(s, ops, hx_, ck_)
                     gt = torch.tensor([g[i] for i in param_ops], dtype=torch.float64)
                     ctx.save_for_backward(gt.to(th.device))

@@ -51,12 +51,21 @@ def hipcc():
 
 
 def sources():
-    return sorted(os.path.join(CSRC, f) for f in os.listdir(CSRC) if f.endswith(".hip"))
+    return sorted(os.path.join(CSRC, f) for f in os.listdir(CSRC)
+                  if f.endswith(".hip") or f.endswith(".cpp"))
+
+
+# Host-only sources (.cpp: the OpenQASM loader) are plain C++ built by the
+# host compiler; they hold no device code.
+HOST = ["-O2", "-fPIC", "-std=c++17", "-Wall", "-I", os.path.join(ROOT, "include")]
 
 
 def _compile(src):
     obj = os.path.join(BUILD, os.path.basename(src) + ".o")
-    cmd = [hipcc(), "-c", src, "-o", obj] + COMMON + EXTRA.get(os.path.basename(src), [])
+    if src.endswith(".cpp"):
+        cmd = [os.environ.get("CXX", "g++"), "-c", src, "-o", obj] + HOST
+    else:
+        cmd = [hipcc(), "-c", src, "-o", obj] + COMMON + EXTRA.get(os.path.basename(src), [])
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"hipcc failed on {src}:\n{r.stdout}\n{r.stderr}")
@@ -67,11 +76,25 @@ def _compile(src):
     return obj
 
 
-def build(verbose=False):
+def _stale(src):
+    """An object is stale when it is older than its source or any header."""
+    obj = os.path.join(BUILD, os.path.basename(src) + ".o")
+    if not os.path.exists(obj):
+        return True
+    deps = [src] + [os.path.join(CSRC, f) for f in os.listdir(CSRC) if f.endswith(".h")]
+    deps += [os.path.join(ROOT, "include", f) for f in os.listdir(os.path.join(ROOT, "include"))]
+    return any(os.path.getmtime(d) > os.path.getmtime(obj) for d in deps)
+
+
+def build(verbose=False, incremental=False):
+    """Compile every source (or, with incremental=True, only those whose
+    object is out of date) and link the library."""
     os.makedirs(BUILD, exist_ok=True)
     srcs = sources()
-    with concurrent.futures.ThreadPoolExecutor(max_workers=min(8, len(srcs))) as ex:
-        objs = list(ex.map(_compile, srcs))
+    todo = [s for s in srcs if _stale(s)] if incremental else srcs
+    with concurrent.futures.ThreadPoolExecutor(max_workers=max(1, min(8, len(todo)))) as ex:
+        list(ex.map(_compile, todo))
+    objs = [os.path.join(BUILD, os.path.basename(s) + ".o") for s in srcs]
     tmp = LIB + ".tmp"
     cmd = [hipcc(), "-shared", "-o", tmp] + objs + [f"--offload-arch={ARCH}", "-fPIC"]
     r = subprocess.run(cmd, capture_output=True, text=True)
@@ -84,4 +107,4 @@ def build(verbose=False):
 
 
 if __name__ == "__main__":
-    build(verbose=True)
+    build(verbose=True, incremental="--incremental" in sys.argv)

@@ -35,9 +35,9 @@ class State:
 
     def __del__(self):
         h = getattr(self, "_h", None)
-        if h is not None and N._lib is not None:
+        if h is not None and N._lib is not None and not N.exiting():
             N._lib.spz_state_destroy(h)
-            self._h = None
+        self._h = None
 
     def __len__(self):
         return 1 << self.n

@@ -505,25 +505,38 @@ int adjoint_reserve(int n, int prec, uint64_t n_ops)
     return sc.buffers(grad_bytes(n_ops), part_bytes(1ull << n));
 }
 
+// The ops a differentiated circuit may hold: unitary single-target gates H..U
+// (and SWAP where allowed) with 1- and 0-controls; every qubit in range, the
+// target (SWAP: both qubits) not a control, no qubit both a 1- and a
+// 0-control, and at most max_controls controls.  The single-circuit and the
+// batched paths share this check, so a circuit is accepted or rejected the
+// same way whichever path runs it.
+int validate_adjoint_op(int n, const spz_op &o, uint64_t idx, bool allow_swap, int max_controls)
+{
+    const bool swap = o.kind == SPZ_SWAP;
+    if (o.kind < SPZ_H || o.kind > SPZ_SWAP || (swap && !allow_swap))
+        return fail(SPZ_ERR_GATE, "adjoint: op " + std::to_string(idx) + " is not a unitary " +
+                                      (allow_swap ? "single-target gate or SWAP"
+                                                  : "single-target gate"));
+    const uint64_t all = n >= 64 ? ~0ull : (1ull << n) - 1;
+    const uint64_t tb = o.target >= 0 && o.target < n ? 1ull << o.target : 0;
+    const uint64_t t1b = o.t1 >= 0 && o.t1 < n ? 1ull << o.t1 : 0;
+    if (!tb || (swap && (!t1b || t1b == tb)) || ((o.ones | o.zeros) & ~all) ||
+        (o.ones & o.zeros) || ((o.ones | o.zeros) & (tb | (swap ? t1b : 0))))
+        return fail(SPZ_ERR_QUBIT, "adjoint: op " + std::to_string(idx) +
+                                       " has a qubit out of range or used twice");
+    if (__builtin_popcountll(o.ones | o.zeros) > max_controls)
+        return fail(SPZ_ERR_GATE, "adjoint: op " + std::to_string(idx) + " has more than " +
+                                      std::to_string(max_controls) + " controls");
+    return SPZ_OK;
+}
+
 int adjoint_grad(spz_state_s *s, const spz_op *ops, uint64_t n_ops, double hx, int ckpt_every,
                  double *grad, double *energy, double *drift)
 {
     SPZ_TRY(check_state(s));
     if (n_ops && (!ops || !grad)) return fail(SPZ_ERR_ARG, "null ops/grad");
-    for (uint64_t i = 0; i < n_ops; i++) {
-        const spz_op &o = ops[i];
-        if (o.kind == SPZ_M || o.kind == SPZ_BITFLIP || o.kind == SPZ_UNITARY ||
-            o.kind < SPZ_H || o.kind > SPZ_SWAP)
-            return fail(SPZ_ERR_GATE, "adjoint: op " + std::to_string(i) +
-                                          " is not a unitary single-target gate or SWAP");
-        const uint64_t all = s->n >= 64 ? ~0ull : (1ull << s->n) - 1;
-        const uint64_t tb = o.target >= 0 && o.target < s->n ? 1ull << o.target : 0;
-        const uint64_t t1b = o.t1 >= 0 && o.t1 < s->n ? 1ull << o.t1 : 0;
-        if (!tb || (o.kind == SPZ_SWAP && (!t1b || t1b == tb)) || ((o.ones | o.zeros) & ~all) ||
-            (o.ones & o.zeros) || ((o.ones | o.zeros) & (tb | (o.kind == SPZ_SWAP ? t1b : 0))))
-            return fail(SPZ_ERR_QUBIT, "adjoint: op " + std::to_string(i) +
-                                           " has a qubit out of range or used twice");
-    }
+    for (uint64_t i = 0; i < n_ops; i++) SPZ_TRY(validate_adjoint_op(s->n, ops[i], i, true, 64));
     const int n = s->n;
     // ---- forward pass with checkpoints ---------------------------------------
     std::vector<std::unique_ptr<OwnedState>> ckpts;
@@ -695,5 +708,6 @@ int adjoint_grad(spz_state_s *s, const spz_op *ops, uint64_t n_ops, double hx, i
 extern "C" int spz_adjoint_grad(spz_state s, const spz_op *ops, uint64_t n_ops, double hx,
                                 int ckpt_every, double *grad, double *energy, double *drift)
 {
+    SPZ_API_LOCK;
     return spz::adjoint_grad(s, ops, n_ops, hx, ckpt_every, grad, energy, drift);
 }

@@ -24,10 +24,12 @@ namespace spz {
 int adjoint_grad(spz_state_s *s, const spz_op *ops, uint64_t n_ops, double hx, int ckpt_every,
                  double *grad, double *energy, double *drift);
 int adjoint_reserve(int n, int prec, uint64_t n_ops);
+int validate_adjoint_op(int n, const spz_op &o, uint64_t idx, bool allow_swap, int max_controls);
 
 namespace {
 
 constexpr int kW = 256;
+constexpr int kMaxControls = 7;  // BOp::pos holds the target and up to 7 controls
 
 // Compact gate record for the persistent kernel.
 struct BOp {
@@ -250,6 +252,7 @@ int spz_batch_adjoint(const spz_batch_circuit *circuits, uint64_t count, const s
                       double hx, double *energies, double *grads, const spz_batch_opts *opts,
                       spz_batch_stats *stats)
 {
+    SPZ_API_LOCK;
     SPZ_TRY(ensure_init());
     if (count == 0) return SPZ_OK;
     if (!circuits || !ops || !energies || !grads) return fail(SPZ_ERR_ARG, "null argument");
@@ -261,10 +264,15 @@ int spz_batch_adjoint(const spz_batch_circuit *circuits, uint64_t count, const s
     for (uint64_t i = 0; i < count; i++) {
         const spz_batch_circuit &c = circuits[i];
         if (c.n < 1 || c.n > 30 || c.n_ops < 0) return fail(SPZ_ERR_ARG, "bad circuit entry");
+        // one rule for both paths (the switch point must not decide whether
+        // a circuit is accepted): gates H..U, at most kMaxControls controls
         for (int o = 0; o < c.n_ops; o++) {
-            const spz_op &op = ops[c.op_offset + o];
-            if (op.target < 0 || op.target >= c.n) return fail(SPZ_ERR_QUBIT, "target out of range");
-            if ((c.n < 64) && ((op.ones | op.zeros) >> c.n)) return fail(SPZ_ERR_QUBIT, "control out of range");
+            const int rc = validate_adjoint_op(c.n, ops[c.op_offset + o], (uint64_t)o, false,
+                                               kMaxControls);
+            if (rc != SPZ_OK) {
+                set_error("circuit " + std::to_string(i) + ": " + spz_last_error());
+                return rc;
+            }
         }
         (c.n <= wg_max ? small : large).push_back(i);
         total_ops += (uint64_t)c.n_ops;
@@ -466,6 +474,7 @@ int spz_batch_autotune(const spz_batch_circuit *circuits, uint64_t count, const 
     // in the batch and `slots` resident workgroups, the persistent kernel needs
     // about ceil(m / slots) * L_wg, the single path m * T_single.  The switch
     // point is the largest n for which the persistent kernel wins.
+    SPZ_API_LOCK;
     SPZ_TRY(ensure_init());
     if (!circuits || !ops) return fail(SPZ_ERR_ARG, "null argument");
     int occ = 0;

@@ -2,7 +2,9 @@
 //
 // PLAIN  : one launch per gate, in program order (the reference's loop).
 // STRICT : runs of consecutive gates are packed into fused tile sweeps
-//          without reordering anything, so results are bit-identical to PLAIN.
+//          without reordering anything, so FP64 results are bit-identical to
+//          PLAIN (FP32 storage: a sweep rounds to FP32 once at its store,
+//          PLAIN after every gate, so they agree within FP32 rounding).
 // AUTO   : like STRICT, but a gate may be pulled ahead of earlier gates it
 //          commutes with (disjoint qubits, or qubits on which both act
 //          diagonally: Z-type targets and controls), which packs more gates

@@ -9,6 +9,7 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -61,6 +62,15 @@ struct Runtime {
 
 Runtime &rt();
 int ensure_init();
+
+// One lock serialises the entry points that use the library's shared state:
+// the reduction scratch and pinned result, the tile-program ring, the dense
+// block fragments, the adjoint scratch, the batched kernel's buffers and the
+// timing records.  Work is ordered on the one library stream anyway, so the
+// lock costs no device parallelism; it makes calls from several threads (on
+// different handles) safe.  Recursive, because entry points call each other.
+std::recursive_mutex &api_mutex();
+#define SPZ_API_LOCK std::lock_guard<std::recursive_mutex> spz_api_lock_(::spz::api_mutex())
 int ensure_partials(size_t pairs);
 
 // Bracket a kernel launch with events when profiling is on.

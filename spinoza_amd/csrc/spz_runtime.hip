@@ -24,6 +24,12 @@ static std::mutex g_init_mu;
 
 Runtime &rt() { return g_rt; }
 
+std::recursive_mutex &api_mutex()
+{
+    static std::recursive_mutex *m = new std::recursive_mutex();  // lives until the process ends
+    return *m;
+}
+
 int ensure_init()
 {
     if (g_rt.ready) return SPZ_OK;
@@ -185,6 +191,7 @@ static int check_qubits(const spz_state_s *s, int target, uint64_t ones, uint64_
 int apply_gate(spz_state_s *s, int kind, double th, double ph, double la, int target, int t1,
                uint64_t ones, uint64_t zeros)
 {
+    SPZ_API_LOCK;
     SPZ_TRY(check_state(s));
     if (kind == SPZ_SWAP) {
         if (target < 0 || target >= s->n || t1 < 0 || t1 >= s->n)
@@ -204,6 +211,7 @@ int apply_gate(spz_state_s *s, int kind, double th, double ph, double la, int ta
 
 int measure(spz_state_s *s, int target, int reset, int forced, double u, uint8_t *out)
 {
+    SPZ_API_LOCK;
     SPZ_TRY(check_state(s));
     if (target < 0 || target >= s->n) return fail(SPZ_ERR_QUBIT, "target qubit out of range");
     if (forced > 1 || forced < -1) return fail(SPZ_ERR_ARG, "forced outcome must be -1, 0 or 1");
@@ -237,6 +245,7 @@ const char *spz_version(void) { return "spinoza_amd 0.1.0 (gfx950)"; }
 
 int spz_device_info(char *name, size_t name_len, int *cu_count, uint64_t *hbm_bytes)
 {
+    SPZ_API_LOCK;
     SPZ_TRY(ensure_init());
     hipDeviceProp_t prop;
     SPZ_HIP(hipGetDeviceProperties(&prop, g_rt.device));
@@ -257,6 +266,7 @@ void *spz_stream(void)
 
 int spz_sync(void)
 {
+    SPZ_API_LOCK;
     SPZ_TRY(ensure_init());
     SPZ_HIP(hipStreamSynchronize(g_rt.stream));
     return SPZ_OK;
@@ -268,6 +278,7 @@ int spz_state_create(int n, int precision, spz_state *out)
     *out = nullptr;
     if (n < 1 || n > 40) return fail(SPZ_ERR_QUBIT, "qubit count must be in [1, 40]");
     if (precision != SPZ_F64 && precision != SPZ_F32) return fail(SPZ_ERR_ARG, "bad precision");
+    SPZ_API_LOCK;
     SPZ_TRY(ensure_init());
     auto *s = new spz_state_s();
     s->n = n;
@@ -299,6 +310,7 @@ int spz_state_create(int n, int precision, spz_state *out)
 int spz_state_destroy(spz_state s)
 {
     if (!s) return SPZ_OK;
+    SPZ_API_LOCK;
     if (g_rt.ready) hipStreamSynchronize(g_rt.stream);
     if (s->re) hipFree(s->re);
     if (s->im) hipFree(s->im);
@@ -308,6 +320,7 @@ int spz_state_destroy(spz_state s)
 
 int spz_state_qubits(spz_state s, int *n)
 {
+    SPZ_API_LOCK;
     SPZ_TRY(check_state(s));
     if (!n) return fail(SPZ_ERR_ARG, "null output");
     *n = s->n;
@@ -316,6 +329,7 @@ int spz_state_qubits(spz_state s, int *n)
 
 int spz_state_precision(spz_state s, int *p)
 {
+    SPZ_API_LOCK;
     SPZ_TRY(check_state(s));
     if (!p) return fail(SPZ_ERR_ARG, "null output");
     *p = s->prec;
@@ -324,6 +338,7 @@ int spz_state_precision(spz_state s, int *p)
 
 int spz_state_set_basis(spz_state s, uint64_t index)
 {
+    SPZ_API_LOCK;
     SPZ_TRY(check_state(s));
     if (index >= s->N) return fail(SPZ_ERR_ARG, "basis index out of range");
     return launch_fill_basis(s, index);
@@ -366,6 +381,7 @@ static int copy_dev(spz_state s, double *dst, const void *src_dev, uint64_t offs
 int spz_state_upload(spz_state s, const double *re, const double *im, uint64_t offset,
                      uint64_t count)
 {
+    SPZ_API_LOCK;
     SPZ_TRY(check_state(s));
     if (offset > s->N || count > s->N - offset) return fail(SPZ_ERR_ARG, "range out of bounds");
     if (count == 0) return SPZ_OK;
@@ -376,6 +392,7 @@ int spz_state_upload(spz_state s, const double *re, const double *im, uint64_t o
 
 int spz_state_download(spz_state s, double *re, double *im, uint64_t offset, uint64_t count)
 {
+    SPZ_API_LOCK;
     SPZ_TRY(check_state(s));
     if (offset > s->N || count > s->N - offset) return fail(SPZ_ERR_ARG, "range out of bounds");
     if (count == 0) return SPZ_OK;
@@ -386,6 +403,7 @@ int spz_state_download(spz_state s, double *re, double *im, uint64_t offset, uin
 
 int spz_state_copy(spz_state dst, spz_state src)
 {
+    SPZ_API_LOCK;
     SPZ_TRY(check_state(dst));
     SPZ_TRY(check_state(src));
     if (dst->n != src->n || dst->prec != src->prec)
@@ -400,6 +418,7 @@ int spz_state_copy(spz_state dst, spz_state src)
 
 int spz_state_device_ptrs(spz_state s, void **re, void **im)
 {
+    SPZ_API_LOCK;
     SPZ_TRY(check_state(s));
     if (re) *re = s->re;
     if (im) *im = s->im;
@@ -408,6 +427,7 @@ int spz_state_device_ptrs(spz_state s, void **re, void **im)
 
 int spz_apply(spz_state s, const spz_gate *g, int target, double eps)
 {
+    SPZ_API_LOCK;
     SPZ_TRY(check_state(s));
     if (!g) return fail(SPZ_ERR_ARG, "null gate");
     if (g->kind == SPZ_BITFLIP) {
@@ -422,6 +442,7 @@ int spz_apply(spz_state s, const spz_gate *g, int target, double eps)
 
 int spz_c_apply(spz_state s, const spz_gate *g, int control, int target)
 {
+    SPZ_API_LOCK;
     SPZ_TRY(check_state(s));
     if (!g) return fail(SPZ_ERR_ARG, "null gate");
     if (control < 0 || control >= s->n) return fail(SPZ_ERR_QUBIT, "control out of range");
@@ -431,6 +452,7 @@ int spz_c_apply(spz_state s, const spz_gate *g, int control, int target)
 
 int spz_cc_apply(spz_state s, const spz_gate *g, int c0, int c1, int target)
 {
+    SPZ_API_LOCK;
     SPZ_TRY(check_state(s));
     if (!g) return fail(SPZ_ERR_ARG, "null gate");
     if (c0 < 0 || c0 >= s->n || c1 < 0 || c1 >= s->n || c0 == c1)
@@ -442,6 +464,7 @@ int spz_cc_apply(spz_state s, const spz_gate *g, int c0, int c1, int target)
 
 int spz_mc_apply(spz_state s, const spz_gate *g, uint64_t ones, uint64_t zeros, int target)
 {
+    SPZ_API_LOCK;
     SPZ_TRY(check_state(s));
     if (!g) return fail(SPZ_ERR_ARG, "null gate");
     if (g->kind > SPZ_U) return fail(SPZ_ERR_GATE, "mc_apply: unsupported gate");
@@ -451,6 +474,7 @@ int spz_mc_apply(spz_state s, const spz_gate *g, uint64_t ones, uint64_t zeros, 
 int spz_transform_u(spz_state s, int m, const double *u_re, const double *u_im, int t,
                     int control)
 {
+    SPZ_API_LOCK;
     SPZ_TRY(check_state(s));
     if (!u_re || !u_im) return fail(SPZ_ERR_ARG, "null matrix");
     if (m < 1 || m > 5) return fail(SPZ_ERR_ARG, "dense block width must be 1..5 qubits");
@@ -474,6 +498,7 @@ int spz_transform_u(spz_state s, int m, const double *u_re, const double *u_im, 
 int spz_execute(spz_state s, const spz_op *ops, uint64_t n_ops, spz_tracker *tracker,
                 const spz_exec_opts *opts, uint8_t *outcomes)
 {
+    SPZ_API_LOCK;
     SPZ_TRY(check_state(s));
     if (n_ops && !ops) return fail(SPZ_ERR_ARG, "null op list");
     return execute_ops(s, ops, n_ops, tracker, opts, outcomes);
@@ -481,6 +506,7 @@ int spz_execute(spz_state s, const spz_op *ops, uint64_t n_ops, spz_tracker *tra
 
 int spz_norm(spz_state s, double *out)
 {
+    SPZ_API_LOCK;
     SPZ_TRY(check_state(s));
     if (!out) return fail(SPZ_ERR_ARG, "null output");
     return reduce(s, nullptr, TERM_NORM, 0, 0, 0.0, out);
@@ -488,6 +514,7 @@ int spz_norm(spz_state s, double *out)
 
 int spz_prob0(spz_state s, int target, double *out)
 {
+    SPZ_API_LOCK;
     SPZ_TRY(check_state(s));
     if (!out) return fail(SPZ_ERR_ARG, "null output");
     if (target < 0 || target >= s->n) return fail(SPZ_ERR_QUBIT, "target qubit out of range");
@@ -501,6 +528,7 @@ int spz_measure_qubit(spz_state s, int target, int reset, int forced, double u, 
 
 int spz_qubit_expectation_value(spz_state s, int target, double *out)
 {
+    if (!out) return fail(SPZ_ERR_ARG, "null output");
     double p0;
     SPZ_TRY(spz_prob0(s, target, &p0));
     *out = 2.0 * p0 - 1.0;  // core.rs:217-218
@@ -510,6 +538,7 @@ int spz_qubit_expectation_value(spz_state s, int target, double *out)
 int spz_xyz_expectation_value(spz_state s, char observable, const int32_t *targets,
                               uint64_t n_targets, double *out)
 {
+    SPZ_API_LOCK;
     SPZ_TRY(check_state(s));
     int term;
     if (observable == 'x') term = TERM_X;
@@ -527,6 +556,7 @@ int spz_xyz_expectation_value(spz_state s, char observable, const int32_t *targe
 
 int spz_expect_zz(spz_state s, int q0, int q1, double *out)
 {
+    SPZ_API_LOCK;
     SPZ_TRY(check_state(s));
     if (!out) return fail(SPZ_ERR_ARG, "null output");
     if (q0 < 0 || q0 >= s->n || q1 < 0 || q1 >= s->n) return fail(SPZ_ERR_QUBIT, "qubit out of range");
@@ -535,6 +565,7 @@ int spz_expect_zz(spz_state s, int q0, int q1, double *out)
 
 int spz_ising_energy(spz_state s, double hx, double *out)
 {
+    SPZ_API_LOCK;
     SPZ_TRY(check_state(s));
     if (!out) return fail(SPZ_ERR_ARG, "null output");
     return reduce(s, nullptr, TERM_ISING, 0, 0, hx, out);
@@ -542,6 +573,7 @@ int spz_ising_energy(spz_state s, double hx, double *out)
 
 int spz_inner(spz_state a, spz_state b, double *re, double *im)
 {
+    SPZ_API_LOCK;
     SPZ_TRY(check_state(a));
     SPZ_TRY(check_state(b));
     if (a->n != b->n) return fail(SPZ_ERR_ARG, "inner product of states of different sizes");
@@ -552,6 +584,7 @@ int spz_inner(spz_state a, spz_state b, double *re, double *im)
 
 int spz_sample(spz_state s, const double *uniforms, uint64_t k, uint64_t *out)
 {
+    SPZ_API_LOCK;
     SPZ_TRY(check_state(s));
     if (k == 0) return SPZ_OK;
     if (!uniforms || !out) return fail(SPZ_ERR_ARG, "null uniforms/output");
@@ -591,6 +624,7 @@ int spz_sample(spz_state s, const double *uniforms, uint64_t k, uint64_t *out)
 
 int spz_profile_enable(int on)
 {
+    SPZ_API_LOCK;
     SPZ_TRY(ensure_init());
     if (!on) resolve_pending();
     g_rt.profiling = on != 0;
@@ -599,6 +633,7 @@ int spz_profile_enable(int on)
 
 int spz_profile_reset(void)
 {
+    SPZ_API_LOCK;
     SPZ_TRY(ensure_init());
     resolve_pending();
     for (int i = 0; i < SPZ_K_COUNT; i++) {
@@ -611,6 +646,7 @@ int spz_profile_reset(void)
 
 int spz_profile_get(int cls, uint64_t *launches, double *total_ms, double *alg_bytes)
 {
+    SPZ_API_LOCK;
     SPZ_TRY(ensure_init());
     if (cls < 0 || cls >= SPZ_K_COUNT) return fail(SPZ_ERR_ARG, "bad kernel class");
     resolve_pending();

@@ -193,6 +193,139 @@ def test_batched_adjoint_matches_single(spz):
             assert np.max(np.abs(g - g1)) < 1e-12
 
 
+@pytest.mark.timeout(600)
+def test_batched_adjoint_config_vs_oracle(spz, orc):
+    """The batched configuration's circuits (RY/RZ + CX-ladder ansatz, n in
+    10..20, depth in 2..12, Ising H with hx = 1) against the oracle: every
+    energy within 1e-12 and two seeded gradient entries per circuit within
+    1e-10 of oracle parameter shift.  wg_max_qubits = 18 puts 10..18 qubits in
+    the persistent one-workgroup kernel and 19..20 on the single-circuit path;
+    the call runs with and without the graph."""
+    print("case batched adjoint vs oracle", flush=True)
+    from spinoza_amd.autograd import batch_adjoint
+    from spinoza_amd.rng import SplitMix64
+    from spinoza_amd.workloads import hea
+    orc.set_threads(16)
+    rng = SplitMix64(5050)
+    circuits, wheres = [], []
+    for n in range(10, 21):
+        for rep in range(2):
+            depth = 2 + rng.next_u64() % 11
+            ops, _, where = hea(n, depth, seed=7000 + 10 * n + rep)
+            circuits.append((n, ops))
+            wheres.append(where)
+    results = [batch_adjoint(circuits, 1.0, wg_max_qubits=18, use_graph=g) for g in (True, False)]
+    for energies, grads, stats in results:
+        assert stats["n_small"] == 18 and stats["n_large"] == 4
+    for c, ((n, ops), where) in enumerate(zip(circuits, wheres)):
+        picks = sorted({where[rng.next_u64() % len(where)] for _ in range(2)})
+        # oracle: energy, then the shifted energies from a shared prefix
+        st = orc.OracleState(n)
+        done = 0
+        ref_g = {}
+        for i in picks:
+            orc.execute(st, ops[done:i])
+            done = i
+            ev = []
+            for sgn in (1.0, -1.0):
+                s2 = st.copy()
+                o = dict(ops[i])
+                o["theta"] += sgn * math.pi / 2
+                orc.execute(s2, [o] + ops[i + 1:])
+                ev.append(s2.ising_energy(1.0))
+            ref_g[i] = 0.5 * (ev[0] - ev[1])
+        orc.execute(st, ops[done:])
+        e_ref = st.ising_energy(1.0)
+        for energies, grads, _ in results:
+            assert abs(energies[c] - e_ref) <= 1e-12, (n, energies[c], e_ref)
+            for i, v in ref_g.items():
+                assert abs(grads[c][i] - v) <= 1e-10, (n, i, grads[c][i], v)
+            others = [k for k in range(len(ops)) if k not in set(where)]
+            assert np.all(grads[c][others] == 0.0)
+
+
+def test_batched_adjoint_rejects_like_single(spz):
+    """One validation rule for both paths: a control on the target, a qubit
+    both a 1- and a 0-control, a SWAP and eight controls are rejected whatever
+    the switch point puts the circuit on."""
+    print("case batched adjoint validation", flush=True)
+    from spinoza_amd import SpinozaError
+    from spinoza_amd import _native as N
+    from spinoza_amd.autograd import adjoint_gradient, batch_adjoint
+    from spinoza_amd.workloads import hea, op
+    good, _, _ = hea(12, 2, seed=3)
+    bad_ops = [op(N.RY, 3, theta=0.4, ones=1 << 3),
+               op(N.RY, 3, theta=0.4, ones=1 << 5, zeros=1 << 5),
+               op(N.SWAP, 2, t1=4),
+               op(N.RZ, 11, theta=0.1, ones=0xFF)]
+    for bad in bad_ops:
+        for wg_max in (20, 4):  # persistent kernel / single-circuit path
+            with pytest.raises(SpinozaError):
+                batch_adjoint([(12, good), (12, good[:5] + [bad] + good[5:])], 1.0,
+                              wg_max_qubits=wg_max)
+    # the library still works after the rejections
+    energies, grads, _ = batch_adjoint([(12, good)], 1.0, wg_max_qubits=20)
+    e1, g1, _ = adjoint_gradient(spz.State(12), good, 1.0)
+    assert abs(energies[0] - e1) < 1e-12 and np.max(np.abs(grads[0] - g1)) < 1e-12
+
+
+def test_adjoint_many_checkpoints_and_error_return(spz, orc):
+    """Regression test for the checkpoint store: with a checkpoint after
+    every op the store grows far past its first capacity, and every restore
+    reads its own state (a store of handles that copied shallowly on growth
+    once freed a checkpoint that a later restore then read).  Then an error
+    return (a bad op past the first ones) leaves the library usable."""
+    print("case adjoint many checkpoints", flush=True)
+    from spinoza_amd import SpinozaError
+    from spinoza_amd import _native as N
+    from spinoza_amd.autograd import adjoint_gradient
+    from spinoza_amd.workloads import hea, op
+    n = 13
+    ops, _, where = hea(n, 3, seed=31)
+    e0, g0, _ = adjoint_gradient(spz.State(n), ops, 1.0, 0)
+    for ck in (1, 2, 5):
+        e1, g1, drift = adjoint_gradient(spz.State(n), ops, 1.0, ck)
+        assert drift < 1e-12
+        assert abs(e1 - e0) < 1e-13 and np.max(np.abs(g1 - g0)) < 1e-12
+    assert abs(e0 - oracle_energy(orc, n, ops)) < 1e-12
+    bad = ops[:40] + [op(N.M, 2)] + ops[40:]
+    with pytest.raises(SpinozaError):
+        adjoint_gradient(spz.State(n), bad, 1.0, 3)
+    bad = ops[:40] + [op(N.RX, n + 3, theta=0.1)] + ops[40:]
+    with pytest.raises(SpinozaError):
+        adjoint_gradient(spz.State(n), bad, 1.0, 3)
+    e2, g2, _ = adjoint_gradient(spz.State(n), ops, 1.0, 4)
+    assert abs(e2 - e0) < 1e-13 and np.max(np.abs(g2 - g0)) < 1e-12
+
+
+def test_states_alive_at_interpreter_exit(tmp_path):
+    """States still referenced when the interpreter exits (a module global, a
+    stored traceback) are left to the process's teardown: the exit is clean."""
+    import os
+    import subprocess
+    import sys
+    print("case exit with live states", flush=True)
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    prog = tmp_path / "live.py"
+    prog.write_text(
+        "import sys\n"
+        f"sys.path.insert(0, {root!r})\n"
+        "import spinoza_amd as spz\n"
+        "from spinoza_amd.autograd import adjoint_gradient\n"
+        "from spinoza_amd.workloads import hea\n"
+        "KEEP = [spz.State(12), spz.State(10, 'f32')]\n"
+        "ops, _, _ = hea(12, 2, seed=1)\n"
+        "adjoint_gradient(KEEP[0], ops, 1.0, 5)\n"
+        "try:\n"
+        "    spz.State(12).copy_from(spz.State(11))\n"
+        "except spz.SpinozaError:\n"
+        "    TB = sys.exc_info()\n"
+        "print('done', flush=True)\n")
+    r = subprocess.run([sys.executable, str(prog)], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, (r.returncode, r.stdout[-2000:], r.stderr[-2000:])
+    assert "done" in r.stdout
+
+
 def test_adjoint_grouped_and_tiled_reverse_sweep(spz, orc):
     """14 qubits: long parameter-free runs (undone by the tile executor), rotation
     groups on up to 3 qubits with controls inside and outside the group, SWAPs
