@@ -19,8 +19,10 @@ Prints ONE JSON line:
              single-circuit calls
   qft        20-qubit QFT of a Gaussian state plus 4096 samples
   per_stride plain-kernel GB/s per target qubit for the gate classes
-  cpu_baseline  the oracle (C restatement of the reference's rayon path,
-             OpenMP) timed on this host on a bounded sample of the workload;
+  cpu_baseline  the reference's per-kind rayon loop shapes restated in C
+             (OpenMP, vectorised; oracle/spz_oracle.c:orc_rayon_*) timed on
+             this host on a bounded sample of the workload, at the host's
+             thread count and on 1 thread;
              the same leg uses the oracle as the checker of the GPU results
              (26-qubit replica, QFT amplitudes and samples, adjoint gradient
              against parameter shift)
@@ -53,6 +55,8 @@ def parse():
     p.add_argument("--no-replica", action="store_true", help="skip the 26q parity replica")
     p.add_argument("--no-strides", action="store_true", help="skip the per-stride sweep")
     p.add_argument("--cpu-threads", type=int, default=0)
+    p.add_argument("--cpu-budget-1t", type=float, default=8.0,
+                   help="seconds of the 1-thread CPU baseline sample")
     p.add_argument("--replica-layers", type=int, default=4)
     p.add_argument("--no-adjoint", action="store_true", help="skip the 24q adjoint leg")
     p.add_argument("--adjoint-qubits", type=int, default=24)
@@ -314,22 +318,33 @@ def qft_leg(spz, N, n=20, seed=1):
 
 
 def cpu_baseline(ops, n, threads, budget_s=20.0):
-    """Time the oracle (OpenMP over the reference's pair loops) on the first
-    gates of the same workload until about `budget_s` of work."""
-    import numpy as np
+    """Time the CPU baseline (the reference's per-kind parallel loop shapes,
+    oracle/spz_oracle.c:orc_rayon_*) on the first gates of the same workload
+    until about `budget_s` of work; returns (gates done, seconds)."""
     from oracle import pyoracle
     pyoracle.set_threads(threads)
     st = pyoracle.OracleState(n)
     done = 0
     t0 = time.perf_counter()
     for o in ops:
-        pyoracle.execute(st, [o])
+        pyoracle.rayon_execute(st, [o])
         done += 1
         if time.perf_counter() - t0 > budget_s:
             break
     dt = time.perf_counter() - t0
     del st
     return done, dt
+
+
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
 
 
 def main():
@@ -450,14 +465,22 @@ def main():
         import numpy as np
         from oracle import pyoracle
         threads = a.cpu_threads or min(16, os.cpu_count() or 1)
-        pyoracle.set_threads(threads)
-        log(f"cpu baseline: oracle, {threads} threads, first gates of the {n}q workload")
+        log(f"cpu baseline: reference loop shapes, {threads} threads, first gates of the "
+            f"{n}q workload")
         done, dt = cpu_baseline(ops, n, threads)
+        log("cpu baseline: 1 thread")
+        done1, dt1 = cpu_baseline(ops, n, 1, budget_s=a.cpu_budget_1t)
+        pyoracle.set_threads(threads)
         result["cpu_baseline"] = {
             "value": round(done / dt, 4), "unit": "gates/s", "cores": threads,
             "kind": "port",
-            "sample": f"first {done} gates of the same {n}q circuit ({dt:.1f} s), "
-                      f"OpenMP restatement of the reference's rayon pair loops"}
+            "sample": f"first {done} of the {len(ops)} gates of the same {n}q circuit "
+                      f"({dt:.1f} s) on a 16 GiB host state: the reference's per-kind "
+                      f"rayon loop shapes (chunk loops, insert-bit controlled loops, "
+                      f"CZ as c_apply(P(pi))) in C with OpenMP, vectorised",
+            "value_1_thread": round(done1 / dt1, 4),
+            "sample_1_thread": f"first {done1} gates ({dt1:.1f} s)",
+            "nproc": os.cpu_count(), "cpu_model": cpu_model()}
         result["speedup_vs_cpu"] = round(value / (done / dt), 1)
         checks = {}
         if not a.no_replica:

@@ -64,6 +64,9 @@ def lib():
         L.orc_get_threads.restype = i
         L.orc_execute.argtypes = [dp, dp, i, ctypes.c_void_p, u64, up]
         L.orc_execute.restype = ctypes.c_int64
+        L.orc_rayon_apply.argtypes = [dp, dp, i, i, d, d, d, i, u64, u64]
+        L.orc_rayon_execute.argtypes = [dp, dp, i, ctypes.c_void_p, u64]
+        L.orc_rayon_execute.restype = ctypes.c_int64
         _lib = L
     return _lib
 
@@ -201,9 +204,7 @@ class _OrcOp(ctypes.Structure):
                 ("u_re", ctypes.c_void_p), ("u_im", ctypes.c_void_p)]
 
 
-def execute(state, ops, tracker=None):
-    """Run an op list (dicts with kind/target/theta/phi/lam/t1/ones/zeros/u/
-    classical) through the oracle's QuantumCircuit::execute restatement."""
+def _encode(ops):
     arr = (_OrcOp * max(1, len(ops)))()
     keep = []
     for i, o in enumerate(ops):
@@ -226,6 +227,25 @@ def execute(state, ops, tracker=None):
             r.m = int(round(np.log2(u.shape[0])))
             r.u_re = ur.ctypes.data
             r.u_im = ui.ctypes.data
+    return arr, keep
+
+
+def rayon_execute(state, ops):
+    """The CPU baseline: an op list of uncontrolled and single-controlled
+    gates through the reference's per-kind parallel loop shapes
+    (orc_rayon_apply).  Results equal execute()'s bit for bit, except CZ,
+    which runs as c_apply(P(pi)) as in the reference."""
+    arr, keep = _encode(ops)
+    rc = lib().orc_rayon_execute(_p(state.re), _p(state.im), state.n, ctypes.addressof(arr),
+                                 len(ops))
+    if rc:
+        raise ValueError(f"CPU baseline does not cover op {-rc - 1}")
+
+
+def execute(state, ops, tracker=None):
+    """Run an op list (dicts with kind/target/theta/phi/lam/t1/ones/zeros/u/
+    classical) through the oracle's QuantumCircuit::execute restatement."""
+    arr, keep = _encode(ops)
     tr = np.zeros(2, dtype=np.uint64) if tracker is None else tracker
     rc = lib().orc_execute(_p(state.re), _p(state.im), state.n, ctypes.addressof(arr), len(ops),
                            tr.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64)))

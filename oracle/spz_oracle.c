@@ -24,7 +24,10 @@
  *   - expectation values (core.rs:198-264) and the build's Ising energy;
  *   - the build's own deterministic reduction tree and inverse-CDF sampler,
  *     restated here operation for operation so GPU results can be compared
- *     bit for bit.
+ *     bit for bit;
+ *   - at the end, the CPU baseline bench.py times (orc_rayon_*): the
+ *     reference's per-kind parallel loop shapes over the same arithmetic,
+ *     checked bit for bit against the checker's gate functions.
  *
  * Parity unpinned: the reference draws measurement outcomes and reservoir
  * samples from an unseeded thread_rng (measurement.rs:35-36 through
@@ -922,6 +925,148 @@ int64_t orc_execute(double *re, double *im, int n, const orc_op *ops,
             rc = orc_mc_apply(re, im, n, o->kind, o->theta, o->phi, o->lambda, ones, o->zeros, o->target, 0);
         }
         if (rc) return -(int64_t)(i + 1);
+    }
+    return 0;
+}
+
+/* ------------------------------------------------------------------------ */
+/* CPU baseline: the reference's parallel loop shapes                        */
+/* ------------------------------------------------------------------------ */
+/* The timed CPU path.  The checker above walks every gate through one
+ * generic pair visitor; the reference instead has one loop per gate kind
+ * (monomorphised Rust, compiled for AVX-512/AVX2 by `multiversion`,
+ * gates.rs:206-214) in two shapes:
+ *  - uncontrolled gates: rayon over the 2^(n-1-t) chunks of 2^(t+1)
+ *    amplitudes, a serial inner loop pairing the low half of a chunk with
+ *    its high half (x_apply gates.rs:346-374, h_apply :600-616, p_apply
+ *    :864-880, z_apply :1075-1088, the RX/RY/U/Y loops); RZ walks chunks of
+ *    2^t multiplying chunk i by diag[i & 1] (rz_apply_strategy1 :930-965);
+ *  - single-controlled gates: rayon over the N/4 pair indices, spread into
+ *    s1 by inserting 1-bits at max(c, t) then min(c, t) (x_c_apply :376-399,
+ *    p_c_apply :882-910); CZ is c_apply(P(pi)) as the reference has no Z arm.
+ * Below gates.rs:14's threshold (15 qubits) the loops run serially.  The
+ * arithmetic is the checker's (the same static inline pair functions), so
+ * results are bit-identical to orc_apply / orc_c_apply (vectorising the
+ * loops changes no rounding: nothing is contracted or reassociated).  The
+ * loops come in AVX-512, AVX2+FMA and baseline versions chosen at run time,
+ * like the reference's multiversion attribute. */
+
+#define ORC_CHUNK_LOOP(BODY)                                                  \
+    do {                                                                      \
+        const uint64_t dist = (uint64_t)1 << t;                               \
+        const int64_t chunks = (int64_t)(((uint64_t)1 << (n - 1)) >> t);      \
+        int64_t ch;                                                           \
+        _Pragma("omp parallel for ORC_PAR(n) schedule(static)")               \
+        for (ch = 0; ch < chunks; ch++) {                                     \
+            const uint64_t base = ((uint64_t)(2 * ch)) << t;                  \
+            for (uint64_t i = 0; i < dist; i++) {                             \
+                const uint64_t s0 = base + i, s1 = s0 + dist;                 \
+                (void)s0; (void)s1;                                           \
+                BODY;                                                         \
+            }                                                                 \
+        }                                                                     \
+    } while (0)
+
+#define ORC_CTRL_LOOP(BODY)                                                   \
+    do {                                                                      \
+        const int64_t end = (int64_t)(((uint64_t)1 << n) >> 2);              \
+        const int lo = c < t ? c : t, hi = c < t ? t : c;                     \
+        const uint64_t dist = (uint64_t)1 << t;                               \
+        int64_t i;                                                            \
+        _Pragma("omp parallel for ORC_PAR(n) schedule(static)")               \
+        for (i = 0; i < end; i++) {                                           \
+            const uint64_t u = (uint64_t)i;                                   \
+            const uint64_t x = u + ((uint64_t)1 << (hi - 1)) +                \
+                               ((u >> (hi - 1)) << (hi - 1));                 \
+            const uint64_t s1 = x + ((uint64_t)1 << lo) + ((x >> lo) << lo);  \
+            const uint64_t s0 = s1 - dist;                                    \
+            (void)s0;                                                         \
+            BODY;                                                             \
+        }                                                                     \
+    } while (0)
+
+/* The loops are compiled three times (AVX-512, AVX2 + FMA, baseline x86-64)
+ * and orc_rayon_apply picks one per call from the CPU's features. */
+#define ORC_TGT __attribute__((target("avx512f,avx512dq,avx512vl,avx2,fma")))
+#define ORC_NAME(f) f##_avx512
+#include "spz_oracle_loops.h"
+#undef ORC_TGT
+#undef ORC_NAME
+#define ORC_TGT __attribute__((target("avx2,fma")))
+#define ORC_NAME(f) f##_avx2
+#include "spz_oracle_loops.h"
+#undef ORC_TGT
+#undef ORC_NAME
+#define ORC_TGT
+#define ORC_NAME(f) f##_base
+#include "spz_oracle_loops.h"
+#undef ORC_TGT
+#undef ORC_NAME
+
+static int orc_isa(void)
+{
+    static int isa = -1;
+    if (isa < 0) {
+        __builtin_cpu_init();
+        isa = (__builtin_cpu_supports("avx512f") && __builtin_cpu_supports("avx512dq") &&
+               __builtin_cpu_supports("avx512vl") && __builtin_cpu_supports("fma")) ? 2
+              : (__builtin_cpu_supports("avx2") && __builtin_cpu_supports("fma")) ? 1 : 0;
+    }
+    return isa;
+}
+
+static void orc_rayon_1q(double *re, double *im, int n, int t, const orc_coef *k)
+{
+    switch (orc_isa()) {
+    case 2: orc_rayon_1q_avx512(re, im, n, t, k); break;
+    case 1: orc_rayon_1q_avx2(re, im, n, t, k); break;
+    default: orc_rayon_1q_base(re, im, n, t, k); break;
+    }
+}
+
+static void orc_rayon_c1(double *re, double *im, int n, int c, int t, const orc_coef *k)
+{
+    switch (orc_isa()) {
+    case 2: orc_rayon_c1_avx512(re, im, n, c, t, k); break;
+    case 1: orc_rayon_c1_avx2(re, im, n, c, t, k); break;
+    default: orc_rayon_c1_base(re, im, n, c, t, k); break;
+    }
+}
+
+/* One gate in the reference's loop shape: uncontrolled single-target gates
+ * and gates with exactly one 1-control.  CZ (Z with one control) runs as the
+ * reference's c_apply(P(pi)); on the checker it is the exact negation, so
+ * the two differ by the residue sin(pi) ~ 1.2e-16 on the touched quarter.
+ * Returns -1 for anything else (SWAP, 0-controls, several controls, M,
+ * noise, dense blocks). */
+int orc_rayon_apply(double *re, double *im, int n, int kind, double th, double ph, double la,
+                    int target, uint64_t ones, uint64_t zeros)
+{
+    if (!orc_check_qubit(n, target) || kind < ORC_H || kind > ORC_U || zeros) return -1;
+    if (ones == 0) {
+        orc_coef k = orc_make_coef(kind, th, ph, la);
+        orc_rayon_1q(re, im, n, target, &k);
+        return 0;
+    }
+    if (__builtin_popcountll(ones) != 1) return -1;
+    int c = __builtin_ctzll(ones);
+    if (!orc_check_qubit(n, c) || c == target) return -1;
+    orc_coef k = kind == ORC_Z ? orc_make_coef(ORC_P, 3.14159265358979323846, 0, 0)
+                               : orc_make_coef(kind, th, ph, la);
+    orc_rayon_c1(re, im, n, c, target, &k);
+    return 0;
+}
+
+/* An op list through orc_rayon_apply (the CPU baseline's execute); returns
+ * 0, or -(index+1) of the first op the loop shapes above do not cover. */
+int64_t orc_rayon_execute(double *re, double *im, int n, const orc_op *ops, uint64_t n_ops)
+{
+    for (uint64_t i = 0; i < n_ops; i++) {
+        const orc_op *o = &ops[i];
+        if (o->kind > ORC_U || o->classical ||
+            orc_rayon_apply(re, im, n, o->kind, o->theta, o->phi, o->lambda, o->target,
+                            o->ones, o->zeros))
+            return -(int64_t)(i + 1);
     }
     return 0;
 }
