@@ -43,6 +43,7 @@
  *
  * Parity status of each entry point is recorded next to it ("pinned by ...").
  */
+#define _GNU_SOURCE /* sincos */
 #include <math.h>
 #include <stdint.h>
 #include <stdlib.h>
@@ -83,42 +84,48 @@ int orc_get_threads(void) { return orc_threads; }
  */
 typedef struct { int kind; double c[8]; } orc_coef;
 
+/* Sine and cosine of one angle from glibc's sincos: the reference computes
+ * each gate's pair with f64::sin_cos (or sin and cos of the same angle side
+ * by side), which rustc emits as LLVM sin/cos intrinsics that LLVM turns
+ * into one sincos call on Linux.  sincos and separate sin/cos calls differ by
+ * one ulp for some angles; the product library uses sincos too. */
+static void orc_sincos(double x, double *s, double *c) { sincos(x, s, c); }
+
 static orc_coef orc_make_coef(int kind, double th, double ph, double la)
 {
     orc_coef k;
+    double s, c;
     memset(&k, 0, sizeof k);
     k.kind = kind;
     switch (kind) {
     case ORC_H:
         k.c[0] = 0.70710678118654752440;   /* std::f64::consts::FRAC_1_SQRT_2 */
         break;
-    case ORC_RX: {
-        double theta = th * 0.5;
-        k.c[0] = cos(theta);
-        k.c[1] = -sin(theta);
+    case ORC_RX:
+        orc_sincos(th * 0.5, &s, &c);
+        k.c[0] = c;
+        k.c[1] = -s;
         break;
-    }
-    case ORC_RY: {
-        double theta = th * 0.5;
-        k.c[0] = sin(theta);
-        k.c[1] = cos(theta);
+    case ORC_RY:
+        orc_sincos(th * 0.5, &s, &c);
+        k.c[0] = s;
+        k.c[1] = c;
         break;
-    }
-    case ORC_RZ: {
-        double theta = th * 0.5;
-        double s = sin(theta), c = cos(theta);
+    case ORC_RZ:
+        orc_sincos(th * 0.5, &s, &c);
         k.c[0] = c; k.c[1] = -s; k.c[2] = c; k.c[3] = s;
         break;
-    }
     case ORC_P:
-        k.c[0] = cos(th);
-        k.c[1] = sin(th);
+        orc_sincos(th, &s, &c);
+        k.c[0] = c;
+        k.c[1] = s;
         break;
     case ORC_U: {
-        double st = sin(th * 0.5), ct = cos(th * 0.5);
-        double sl = sin(la), cl = cos(la);
-        double spl = sin(ph + la), cpl = cos(ph + la);
-        double sp = sin(ph), cp = cos(ph);
+        double st, ct, sl, cl, spl, cpl, sp, cp;
+        orc_sincos(th * 0.5, &st, &ct);
+        orc_sincos(la, &sl, &cl);
+        orc_sincos(ph + la, &spl, &cpl);
+        orc_sincos(ph, &sp, &cp);
         k.c[0] = ct;         k.c[1] = 0.0;
         k.c[2] = -cl * st;   k.c[3] = -sl * st;
         k.c[4] = cp * st;    k.c[5] = sp * st;

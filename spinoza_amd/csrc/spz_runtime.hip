@@ -118,6 +118,13 @@ LaunchTimer::~LaunchTimer()
 // ---------------------------------------------------------------------------
 bool is_diagonal(int ak) { return ak == AK_NEG || ak == AK_PHASE || ak == AK_RZ; }
 
+// Sine and cosine of one angle from glibc's sincos.  The reference computes
+// every gate's pair with f64::sin_cos, or sin and cos of the same angle side
+// by side; rustc emits LLVM's sin/cos intrinsics, which LLVM turns into one
+// sincos call on Linux.  sincos and separate sin/cos calls differ by one ulp
+// for some angles, so the coefficients come from sincos, as in the oracle.
+static inline void sin_cos(double x, double *s, double *c) { ::sincos(x, s, c); }
+
 int make_coef(int kind, double th, double ph, double la, Coef *k)
 {
     memset(k, 0, sizeof *k);
@@ -129,37 +136,43 @@ int make_coef(int kind, double th, double ph, double la, Coef *k)
     case SPZ_X: k->ak = AK_X; return SPZ_OK;
     case SPZ_Y: k->ak = AK_Y; return SPZ_OK;
     case SPZ_Z: k->ak = AK_NEG; return SPZ_OK;
-    case SPZ_P:  // p_apply: let (sin, cos) = sin_cos(angle) (gates.rs:865)
+    case SPZ_P: {  // p_apply: let (sin, cos) = sin_cos(angle) (gates.rs:865)
+        double s, c;
+        sin_cos(th, &s, &c);
         k->ak = AK_PHASE;
-        k->c[0] = cos(th);
-        k->c[1] = sin(th);
+        k->c[0] = c;
+        k->c[1] = s;
         return SPZ_OK;
+    }
     case SPZ_RX: {  // rx_apply: theta = angle*0.5; cos, -sin (gates.rs:749-751)
-        double t = th * 0.5;
+        double s, c;
+        sin_cos(th * 0.5, &s, &c);
         k->ak = AK_RX;
-        k->c[0] = cos(t);
-        k->c[1] = -sin(t);
+        k->c[0] = c;
+        k->c[1] = -s;
         return SPZ_OK;
     }
     case SPZ_RY: {  // ry_apply: (sin, cos) = sin_cos(angle*0.5) (gates.rs:1125-1126)
-        double t = th * 0.5;
+        double s, c;
+        sin_cos(th * 0.5, &s, &c);
         k->ak = AK_RY;
-        k->c[0] = sin(t);
-        k->c[1] = cos(t);
+        k->c[0] = s;
+        k->c[1] = c;
         return SPZ_OK;
     }
     case SPZ_RZ: {  // rz_apply: d0 = (c, -s), d1 = (c, s) (gates.rs:969-975)
-        double t = th * 0.5;
-        double s = sin(t), c = cos(t);
+        double s, c;
+        sin_cos(th * 0.5, &s, &c);
         k->ak = AK_RZ;
         k->c[0] = c; k->c[1] = -s; k->c[2] = c; k->c[3] = s;
         return SPZ_OK;
     }
     case SPZ_U: {  // u_apply (gates.rs:1286-1303)
-        double st = sin(th * 0.5), ct = cos(th * 0.5);
-        double sl = sin(la), cl = cos(la);
-        double spl = sin(ph + la), cpl = cos(ph + la);
-        double sp = sin(ph), cp = cos(ph);
+        double st, ct, sl, cl, spl, cpl, sp, cp;
+        sin_cos(th * 0.5, &st, &ct);
+        sin_cos(la, &sl, &cl);
+        sin_cos(ph + la, &spl, &cpl);
+        sin_cos(ph, &sp, &cp);
         k->ak = AK_U;
         k->c[0] = ct; k->c[1] = 0.0;
         k->c[2] = -cl * st; k->c[3] = -sl * st;
