@@ -216,6 +216,16 @@ int spz_xyz_expectation_value(spz_state s, char observable, const int32_t *targe
 int spz_expect_zz(spz_state s, int q0, int q1, double *out);
 /* <H> for H = sum_{i<n-1} Z_i Z_{i+1} + hx * sum_i X_i (open chain). */
 int spz_ising_energy(spz_state s, double hx, double *out);
+/* spz_ising_energy without allocation.  No counterpart in the reference: it
+ * is for callers that cannot allocate device or pinned memory while they run
+ * (tight serving and optimiser loops, code that forbids hipMalloc).  The
+ * block partials live in a static device array of 4 MiB that the library
+ * holds until the process ends, so states are limited to 2^30 amplitudes
+ * (larger ones fail with SPZ_ERR_ARG).  The result is bit-identical to
+ * spz_ising_energy.  The array is shared by every call in the process: the
+ * library's lock serialises them, so calls from several threads do not
+ * overlap on the device. */
+int spz_ising_energy_noalloc(spz_state s, double hx, double *out);
 /* <a|b> for two states of equal size; their storage precisions may differ
    (an FP32 state against an FP64 one: the terms are formed and summed in FP64). */
 int spz_inner(spz_state a, spz_state b, double *re, double *im);

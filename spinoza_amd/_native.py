@@ -120,6 +120,7 @@ SIGNATURES = {
                                        _u64, _dp]),
     "spz_expect_zz": (_i, [_vp, _i, _i, _dp]),
     "spz_ising_energy": (_i, [_vp, _d, _dp]),
+    "spz_ising_energy_noalloc": (_i, [_vp, _d, _dp]),
     "spz_inner": (_i, [_vp, _vp, _dp, _dp]),
     "spz_sample": (_i, [_vp, _dp, _u64, _u64p]),
     "spz_adjoint_grad": (_i, [_vp, ctypes.POINTER(Op_t), _u64, _d, _i, _dp, _dp, _dp]),

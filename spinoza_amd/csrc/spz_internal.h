@@ -269,6 +269,10 @@ int reduce(const spz_state_s *a, const spz_state_s *b, int term, int q0, int q1,
            double *out);
 // Level-1 block partials of |psi|^2 (for sampling); returns #blocks.
 int norm_block_partials(const spz_state_s *s, std::vector<double> &W);
+// The Ising energy through fixed-size static scratch (no allocation); the
+// scratch's device address is taken by static_scratch_init at library init.
+int static_scratch_init();
+int ising_energy_noalloc(const spz_state_s *s, double hx, double *out);
 int launch_sample(const spz_state_s *s, const double *d_prefix, uint64_t nb, int64_t lastnz,
                   const double *d_u, uint64_t k, uint64_t *d_out);
 

@@ -619,12 +619,11 @@ int launch_dense(spz_state_s *s, int m, const double *dur, const double *dui, in
 // ---------------------------------------------------------------------------
 template <typename T, typename T2 = T>
 static int reduce_launch(const spz_state_s *a, const spz_state_s *b, int term, int q0, int q1,
-                         double hx, uint64_t M, uint64_t nb)
+                         double hx, uint64_t M, uint64_t nb, double *part)
 {
     const T *re = (const T *)a->re, *im = (const T *)a->im;
     const T2 *re2 = b ? (const T2 *)b->re : nullptr, *im2 = b ? (const T2 *)b->im : nullptr;
     hipStream_t st = rt().stream;
-    double *part = rt().d_partials;
     dim3 g((unsigned)nb), blk(kThreads);
     int n = a->n;
 #define SPZ_RL(TM) hipLaunchKernelGGL((k_reduce1<T, T2, TM>), g, blk, 0, st, re, im, re2, im2, M, q0, q1, n, hx, part)
@@ -663,12 +662,13 @@ static int reduce_level1(const spz_state_s *a, const spz_state_s *b, int term, i
     if (term == TERM_DOT_RE || term == TERM_DOT_IM) per_term = amp_bytes(a) + amp_bytes(b);
     LaunchTimer tm(SPZ_K_REDUCE, (double)M * per_term);
     int rc;
+    double *part = rt().d_partials;
     if (b && b->prec != a->prec)
-        rc = a->prec == SPZ_F64 ? reduce_launch<double, float>(a, b, term, q0, q1, hx, M, nb)
-                                : reduce_launch<float, double>(a, b, term, q0, q1, hx, M, nb);
+        rc = a->prec == SPZ_F64 ? reduce_launch<double, float>(a, b, term, q0, q1, hx, M, nb, part)
+                                : reduce_launch<float, double>(a, b, term, q0, q1, hx, M, nb, part);
     else
-        rc = a->prec == SPZ_F64 ? reduce_launch<double>(a, b, term, q0, q1, hx, M, nb)
-                                : reduce_launch<float>(a, b, term, q0, q1, hx, M, nb);
+        rc = a->prec == SPZ_F64 ? reduce_launch<double>(a, b, term, q0, q1, hx, M, nb, part)
+                                : reduce_launch<float>(a, b, term, q0, q1, hx, M, nb, part);
     *nb_out = nb;
     return rc;
 }
@@ -686,6 +686,56 @@ int reduce(const spz_state_s *a, const spz_state_s *b, int term, int q0, int q1,
                            rt().d_result);
     SPZ_HIP(hipGetLastError());
     SPZ_HIP(hipMemcpyAsync(rt().h_result, rt().d_result, sizeof(double), hipMemcpyDeviceToHost, st));
+    SPZ_HIP(hipStreamSynchronize(st));
+    *out = rt().h_result[0];
+    return SPZ_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Ising energy without allocation (spz_ising_energy_noalloc)
+// ---------------------------------------------------------------------------
+// The block partials and the result live in a file-scope device array of fixed
+// size instead of the grow-only scratch that reduce() allocates: 2^18 blocks
+// of 4096 terms (states up to 2^30 amplitudes, 30 qubits), 4 MiB of partial
+// pairs plus the result.  Host code reaches it only through the address that
+// hipGetSymbolAddress returns (the array's name is a host-side stand-in);
+// that address is taken once, when the library initialises on its device.
+// Every call shares the array; the library lock serialises calls within a
+// process, and each call overwrites every pair it reads (no clearing needed).
+constexpr uint64_t kStaticBlocks = 1ull << 18;
+constexpr uint64_t kStaticResult = 2 * kStaticBlocks;  // index of the result slot
+static __device__ __align__(256) double scratch[2 * kStaticBlocks + 32];
+static double *g_static_scratch = nullptr;
+
+int static_scratch_init()
+{
+    void *p = nullptr;
+    SPZ_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    g_static_scratch = static_cast<double *>(p);
+    return SPZ_OK;
+}
+
+int ising_energy_noalloc(const spz_state_s *s, double hx, double *out)
+{
+    if (!g_static_scratch) return fail(SPZ_ERR_ARG, "library not initialised");
+    const uint64_t M = s->N;
+    const uint64_t nb = (M + kRB - 1) / kRB;
+    if (nb > kStaticBlocks)
+        return fail(SPZ_ERR_ARG, "state too large for spz_ising_energy_noalloc (at most 2^30 "
+                                 "amplitudes; use spz_ising_energy)");
+    double *part = g_static_scratch, *res = g_static_scratch + kStaticResult;
+    const int rc = s->prec == SPZ_F64
+                       ? reduce_launch<double>(s, nullptr, TERM_ISING, 0, 0, hx, M, nb, part)
+                       : reduce_launch<float>(s, nullptr, TERM_ISING, 0, 0, hx, M, nb, part);
+    SPZ_TRY(rc);
+    hipStream_t st = rt().stream;
+    if (nb == 1)
+        hipLaunchKernelGGL(k_finish1, dim3(1), dim3(1), 0, st, part, res);
+    else
+        hipLaunchKernelGGL(k_reduce2, dim3(1), dim3(kThreads), 0, st, part, nb, res);
+    SPZ_HIP(hipGetLastError());
+    // the pinned result slot was allocated when the library initialised
+    SPZ_HIP(hipMemcpyAsync(rt().h_result, res, sizeof(double), hipMemcpyDeviceToHost, st));
     SPZ_HIP(hipStreamSynchronize(st));
     *out = rt().h_result[0];
     return SPZ_OK;

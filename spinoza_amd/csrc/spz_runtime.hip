@@ -46,6 +46,7 @@ int ensure_init()
     SPZ_HIP(hipStreamCreate(&g_rt.stream));
     SPZ_HIP(hipMalloc(&g_rt.d_result, 4 * sizeof(double)));
     SPZ_HIP(hipHostMalloc(&g_rt.h_result, 4 * sizeof(double), hipHostMallocDefault));
+    SPZ_TRY(static_scratch_init());
     g_rt.ready = true;
     return SPZ_OK;
 }
@@ -582,6 +583,15 @@ int spz_ising_energy(spz_state s, double hx, double *out)
     SPZ_TRY(check_state(s));
     if (!out) return fail(SPZ_ERR_ARG, "null output");
     return reduce(s, nullptr, TERM_ISING, 0, 0, hx, out);
+}
+
+int spz_ising_energy_noalloc(spz_state s, double hx, double *out)
+{
+    SPZ_API_LOCK;
+    SPZ_TRY(check_state(s));
+    if (!out) return fail(SPZ_ERR_ARG, "null output");
+    SPZ_TRY(ensure_init());
+    return ising_energy_noalloc(s, hx, out);
 }
 
 int spz_inner(spz_state a, spz_state b, double *re, double *im)
