@@ -38,8 +38,10 @@ COMMON = [
 # slower).  Keeping the cases apart keeps them in VGPRs.
 # The tile kernel also asks for a 6-waves-per-SIMD register budget (see its
 # source); the compiler reaches 4 without spills and warns that 6 was not met.
+_TILE_FLAGS = ["-mllvm", "-simplifycfg-sink-common=false", "-Wno-pass-failed"]
 EXTRA = {
-    "spz_tile.hip": ["-mllvm", "-simplifycfg-sink-common=false", "-Wno-pass-failed"],
+    "spz_tile.hip": _TILE_FLAGS,       # exact arithmetic
+    "spz_tile_fast.hip": _TILE_FLAGS,  # AUTO arithmetic (compiled in parallel)
 }
 
 

@@ -159,10 +159,22 @@ uint32_t kind_bit(int ak)
     }
 }
 
+// A change of register layout inside a sweep: an LDS transpose to any
+// layout, or in-wavefront swaps of register-index bits with lane bits 4 and 5
+// (v_permlane16/32_swap), which move no data through LDS.
+struct Change {
+    bool lds = true;
+    std::vector<int> rs;                    // register slot -> local bit after it
+    std::vector<int> thr;                   // thread-index bit -> local bit after it
+    std::vector<std::pair<int, int>> sw;    // (register slot, lane bit), in order
+};
+
 struct Planner {
     int n, K, LOGW, RB, klo;
     int improve = 0;  // local-search passes over the tile set (AUTO only)
     bool strict;
+    bool fast = false;   // AUTO arithmetic (see launch_tile): H factors carried
+    bool swaps = true;   // in-wavefront layout changes when they reach the set
     bool bad = false;
     std::vector<TileOp> prog;
     std::vector<Sweep> sweeps;
@@ -263,25 +275,92 @@ struct Planner {
         }
     }
 
-    void emit_layout(const std::vector<int> &oldRS, const std::vector<int> &RSv)
+    void emit_layout(const std::vector<int> &oldRS, const std::vector<int> &RSv,
+                     const std::vector<int> &thr)
     {
         TileOp op;
         memset(&op, 0, sizeof op);
         op.type = TOP_LAYOUT;
         op.grp = TG_LAYOUT;
         op.eo = 1;  // em = 0: never taken for a gate
-        uint32_t used = 0;
         for (int j = 0; j < RB; j++) {
             op.lay.oldreg[j] = (int8_t)oldRS[j];
             op.lay.newreg[j] = (int8_t)RSv[j];
             op.lay.rob[j] = tile_swz(1u << oldRS[j]) << 3;
             op.lay.rnb[j] = tile_swz(1u << RSv[j]) << 3;
-            used |= 1u << RSv[j];
         }
-        int i = 0;
-        for (int b = 0; b < K; b++)
-            if (!((used >> b) & 1)) op.lay.newthr[i++] = (int8_t)b;
+        for (int i = 0; i < LOGW; i++) op.lay.newthr[i] = (int8_t)thr[i];
         prog.push_back(op);
+    }
+
+    // Register slot j trades its local bit with the one on lane bit tb (4 or 5).
+    void emit_swap(int j, int tb, int reg_bit, int lane_bit)
+    {
+        TileOp op;
+        memset(&op, 0, sizeof op);
+        op.type = TOP_LAYOUT;
+        op.grp = tb == 5 ? TG_SW32 : TG_SW16;
+        op.eo = 1;  // never taken for a gate
+        op.sel = 1u << j;
+        op.mlo = 1u << lane_bit;  // leaves the lane part
+        op.mlz = 1u << reg_bit;   // enters it
+        prog.push_back(op);
+    }
+
+    // Thread layout after an LDS change to register bits rs: lane bits 4 and
+    // 5 take the two best-scored free bits (the next register candidates, so
+    // that later changes can be swaps); the other free bits fill the thread
+    // bits in ascending order, low bits first.  (lo: bits below it may not
+    // go to lanes 4, 5, used for the load layout, whose lanes 0..klo-1 must
+    // hold the low bits.)
+    std::vector<int> thread_layout(const std::vector<int> &rs, const std::vector<long> *score,
+                                   int lo) const
+    {
+        std::vector<int> avail;
+        for (int b = 0; b < K; b++)
+            if (std::find(rs.begin(), rs.end(), b) == rs.end()) avail.push_back(b);
+        std::vector<int> thr(LOGW, -1);
+        if (swaps && LOGW >= 6 && score) {
+            std::vector<int> cand;
+            for (int b : avail)
+                if (b >= lo && (*score)[b] > 0) cand.push_back(b);
+            std::stable_sort(cand.begin(), cand.end(),
+                             [&](int x, int y) { return (*score)[x] > (*score)[y]; });
+            for (int i = 0; i < 2 && i < (int)cand.size(); i++) {
+                thr[4 + i] = cand[i];
+                avail.erase(std::find(avail.begin(), avail.end(), cand[i]));
+            }
+        }
+        size_t a = 0;
+        for (int i = 0; i < LOGW; i++)
+            if (thr[i] < 0) thr[i] = avail[a++];
+        return thr;
+    }
+
+    // Swaps that turn register set rs (slots) into one holding exactly the
+    // bits of `want`, if every missing bit sits on lane bit 4 or 5.
+    bool swap_path(const std::vector<int> &rs, const std::vector<int> &thr,
+                   const std::vector<int> &want, std::vector<std::pair<int, int>> *sw) const
+    {
+        if (!swaps || LOGW < 6) return false;
+        std::vector<int> in, out;
+        for (int b : want)
+            if (std::find(rs.begin(), rs.end(), b) == rs.end()) in.push_back(b);
+        for (int j = 0; j < RB; j++)
+            if (std::find(want.begin(), want.end(), rs[j]) == want.end()) out.push_back(j);
+        if (in.size() != out.size() || in.size() > 2) return false;
+        for (size_t i = 0; i < in.size(); i++) {
+            const int tb = thr[4] == in[i] ? 4 : thr[5] == in[i] ? 5 : -1;
+            if (tb < 0) return false;
+            if (sw) sw->push_back({out[i], tb});
+        }
+        return true;
+    }
+
+    void apply_swaps(std::vector<int> &rs, std::vector<int> &thr,
+                     const std::vector<std::pair<int, int>> &sw) const
+    {
+        for (auto [j, tb] : sw) std::swap(rs[j], thr[tb]);
     }
 
     // Dependency graph of a sweep's gates: a -> b (a before b) when they share
@@ -331,10 +410,16 @@ struct Planner {
     // The RB register bits (from `lo` up) to switch to: every RB-subset of
     // the (up to 8) best-scored bits is tried, and the one under which the
     // most gates run wins (ties: the higher score, then the earlier subset).
+    // With a current layout (cur_rs, cur_thr), a set reachable by in-wavefront
+    // swaps is charged less than one that needs an LDS transpose (in gates
+    // that could have run: a transpose costs about four gates, a swap about
+    // half of one).
     std::vector<int> best_set(const std::vector<SG> &run, const std::vector<int> &order,
                               const int *lbit, const std::vector<std::vector<int>> &succ,
                               const std::vector<int> &indeg, const std::vector<char> &done,
-                              const std::vector<long> &score, int lo) const
+                              const std::vector<long> &score, int lo,
+                              const std::vector<int> *cur_rs = nullptr,
+                              const std::vector<int> *cur_thr = nullptr) const
     {
         int left = 0;
         for (char d : done) left += !d;
@@ -344,28 +429,45 @@ struct Planner {
                          [&](int x, int y) { return score[x] > score[y]; });
         const int T = std::min((int)bits.size(), std::max(RB, 8));
         uint32_t best = 0;
-        int best_cnt = -1;
+        double best_val = -1e30;
         long best_sc = -1;
         for (uint32_t sub = 0; sub < (1u << T); sub++) {
             if (__builtin_popcount(sub) != RB) continue;
             uint32_t rs = 0;
             long sc = 0;
+            std::vector<int> want;
             for (int i = 0; i < T; i++)
                 if ((sub >> i) & 1) {
                     rs |= 1u << bits[i];
                     sc += score[bits[i]];
+                    want.push_back(bits[i]);
                 }
             int cnt = runs_under(run, order, lbit, succ, indeg, done, rs);
             // a set that finishes the sweep and keeps the low bits on lanes
             // saves the change back before the store
             if (cnt == left && !(rs & ((1u << klo) - 1))) cnt++;
-            if (cnt > best_cnt || (cnt == best_cnt && sc > best_sc)) {
+            // a set under which nothing runs makes no progress
+            if (cnt == 0) continue;
+            double val = cnt;
+            if (cur_rs && cur_thr) {
+                std::vector<std::pair<int, int>> sw;
+                val -= swap_path(*cur_rs, *cur_thr, want, &sw) ? 0.5 * (double)sw.size() : 4.0;
+            }
+            if (val > best_val || (val == best_val && sc > best_sc)) {
                 best = rs;
-                best_cnt = cnt;
+                best_val = val;
                 best_sc = sc;
             }
         }
         std::vector<int> out;
+        if (best_val == -1e30) {
+            // nothing runs under any candidate (cannot happen while a ready
+            // pair gate's target is among the best-scored bits): take the top
+            // bits, which hold every ready target
+            for (int i = 0; i < RB && i < (int)bits.size(); i++) out.push_back(bits[i]);
+            std::sort(out.begin(), out.end());
+            return out;
+        }
         for (int b = 0; b < K; b++)
             if ((best >> b) & 1) out.push_back(b);
         return out;
@@ -374,14 +476,15 @@ struct Planner {
     // Sequence the sweep's gates: the scheduled order itself (strict), or,
     // with reordering allowed, a topological order of the dependency graph
     // that applies every gate the current register layout allows before
-    // switching layouts.  Entries >= 0 are gates; -1 marks "switch layout to
-    // the next entry of `layouts`".  rs_in receives the register bits of the
-    // load layout: never one of the klo low local bits (they stay on the low
-    // lane bits, for contiguous loads); with reordering allowed, the bits
-    // with the most ready pair gates.
+    // switching layouts.  Entries >= 0 are gates; -1 marks "apply the next
+    // entry of `changes`".  rs_in / thr_in receive the load layout: register
+    // bits never among the klo low local bits, which stay on the low lane bits
+    // for contiguous loads (with reordering allowed, the register bits with
+    // the most ready pair gates, and lane bits 4 and 5 holding the next
+    // best-scored bits).
     void sequence(const std::vector<SG> &run, const std::vector<int> &order, const int *lbit,
-                  std::vector<int> &seq, std::vector<std::vector<int>> &layouts,
-                  std::vector<int> &rs_in)
+                  std::vector<int> &seq, std::vector<Change> &changes, std::vector<int> &rs_in,
+                  std::vector<int> &thr_in)
     {
         const int m = (int)order.size();
         std::vector<int> RS(RB);
@@ -391,57 +494,90 @@ struct Planner {
                 if (RS[j] == b) return true;
             return false;
         };
-        if (!strict && K - klo >= RB) {
-            std::vector<std::vector<int>> succ0;
-            std::vector<int> indeg0;
-            deps_of(run, order, succ0, indeg0);
+        // pair-gate targets still to run, weighted by readiness
+        auto pair_score = [&](const std::vector<int> &indeg, const std::vector<char> *done) {
             std::vector<long> score(K, 0);
             for (int a = 0; a < m; a++) {
+                if (done && (*done)[a]) continue;
                 const SG &g = run[order[a]];
-                if (!g.diag) score[lbit[g.target]] += indeg0[a] == 0 ? 1000 : 1;
+                if (!g.diag) score[lbit[g.target]] += indeg[a] == 0 ? 1000 : 1;
             }
-            RS = best_set(run, order, lbit, succ0, indeg0, std::vector<char>(m, 0), score, klo);
-        }
+            return score;
+        };
+        std::vector<std::vector<int>> succ0;
+        std::vector<int> indeg0;
+        deps_of(run, order, succ0, indeg0);
+        std::vector<long> score0 = pair_score(indeg0, nullptr);
+        if (!strict && K - klo >= RB)
+            RS = best_set(run, order, lbit, succ0, indeg0, std::vector<char>(m, 0), score0, klo);
+        std::vector<int> THR = thread_layout(RS, &score0, klo);
         rs_in = RS;
+        thr_in = THR;
+        // move to register bits `nx` (slot order as given): swaps when they
+        // reach it, else an LDS transpose whose lane bits 4, 5 get the
+        // best-scored free bits
+        auto change_to = [&](const std::vector<int> &nx, const std::vector<long> &score) {
+            Change ch;
+            if (swap_path(RS, THR, nx, &ch.sw)) {
+                ch.lds = false;
+                apply_swaps(RS, THR, ch.sw);
+            } else {
+                // keep bits that stay in place in their old slots
+                std::vector<int> slotted(RB, -1);
+                for (int j = 0; j < RB; j++)
+                    if (std::find(nx.begin(), nx.end(), RS[j]) != nx.end()) slotted[j] = RS[j];
+                for (int b : nx) {
+                    if (std::find(slotted.begin(), slotted.end(), b) != slotted.end()) continue;
+                    for (int j = 0; j < RB; j++)
+                        if (slotted[j] < 0) { slotted[j] = b; break; }
+                }
+                RS = slotted;
+                THR = thread_layout(RS, &score, 0);
+            }
+            ch.rs = RS;
+            ch.thr = THR;
+            seq.push_back(-1);
+            changes.push_back(ch);
+        };
         if (strict) {
             for (int pos = 0; pos < m; pos++) {
                 const SG &g = run[order[pos]];
                 if (!g.diag && !in_rs(lbit[g.target])) {
                     std::vector<int> nx;
-                    for (int p2 = pos; p2 < m && (int)nx.size() < RB; p2++) {
+                    std::vector<long> score(K, 0);
+                    for (int p2 = pos; p2 < m; p2++) {
                         const SG &h = run[order[p2]];
                         if (h.diag) continue;
                         int hb = lbit[h.target];
-                        if (std::find(nx.begin(), nx.end(), hb) == nx.end()) nx.push_back(hb);
+                        score[hb] += std::max(1, 64 - (p2 - pos));
+                        if ((int)nx.size() < RB && std::find(nx.begin(), nx.end(), hb) == nx.end())
+                            nx.push_back(hb);
                     }
                     for (int j = 0; j < RB && (int)nx.size() < RB; j++)
                         if (std::find(nx.begin(), nx.end(), RS[j]) == nx.end()) nx.push_back(RS[j]);
                     for (int bb = 0; bb < K && (int)nx.size() < RB; bb++)
                         if (std::find(nx.begin(), nx.end(), bb) == nx.end()) nx.push_back(bb);
-                    RS = nx;
-                    seq.push_back(-1);
-                    layouts.push_back(RS);
+                    change_to(nx, score);
                 }
                 seq.push_back(order[pos]);
             }
-            // a first layout change ahead of every gate that keeps the low
-            // bits on lanes becomes the load layout
-            if (!seq.empty() && seq[0] == -1) {
+            // a first LDS change ahead of every gate that keeps the low bits
+            // on lanes becomes the load layout
+            if (!seq.empty() && seq[0] == -1 && changes[0].lds) {
                 bool low = false;
-                for (int b : layouts[0]) low |= b < klo;
+                for (int b : changes[0].rs) low |= b < klo;
+                for (int i = 0; i < klo && i < LOGW; i++) low |= changes[0].thr[i] != i;
                 if (!low) {
-                    rs_in = layouts[0];
+                    rs_in = changes[0].rs;
+                    thr_in = changes[0].thr;
                     seq.erase(seq.begin());
-                    layouts.erase(layouts.begin());
+                    changes.erase(changes.begin());
                 }
             }
             return;
         }
-        // dependency graph: i -> j (i before j) when they share a qubit on
-        // which at least one acts non-diagonally
-        std::vector<std::vector<int>> succ;
-        std::vector<int> indeg;
-        deps_of(run, order, succ, indeg);
+        std::vector<std::vector<int>> &succ = succ0;
+        std::vector<int> &indeg = indeg0;
         std::vector<char> done(m, 0);
         int left = m;
         while (left > 0) {
@@ -463,27 +599,14 @@ struct Planner {
             // next layout: score target bits of the remaining non-diagonal
             // gates, ready ones first; among the RB-subsets of the best
             // scored bits, take the one under which the most gates run
-            // before the next change (ties: the higher score)
-            std::vector<long> score(K, 0);
-            for (int a = 0; a < m; a++) {
-                if (done[a]) continue;
-                const SG &g = run[order[a]];
-                if (g.diag) continue;
-                score[lbit[g.target]] += indeg[a] == 0 ? 1000 : 1;
+            // before the next change, less the cost of reaching it
+            std::vector<long> score = pair_score(indeg, &done);
+            std::vector<int> nx = best_set(run, order, lbit, succ, indeg, done, score, 0, &RS, &THR);
+            change_to(nx, score);
+            if ((int)changes.size() > 2 * m + 8) {  // every change runs a gate: cannot happen
+                bad = true;
+                return;
             }
-            std::vector<int> nx = best_set(run, order, lbit, succ, indeg, done, score, 0);
-            // keep bits that stay in place in their old slots
-            std::vector<int> slotted(RB, -1);
-            for (int j = 0; j < RB; j++)
-                if (std::find(nx.begin(), nx.end(), RS[j]) != nx.end()) slotted[j] = RS[j];
-            for (int b : nx) {
-                if (std::find(slotted.begin(), slotted.end(), b) != slotted.end()) continue;
-                for (int j = 0; j < RB; j++)
-                    if (slotted[j] < 0) { slotted[j] = b; break; }
-            }
-            RS = slotted;
-            seq.push_back(-1);
-            layouts.push_back(RS);
         }
     }
 
@@ -503,10 +626,10 @@ struct Planner {
             }
         for (; i < 16; i++) sw.desc.spos[i] = 64;
         sw.op_begin = prog.size();
-        std::vector<int> seq, RS;
-        std::vector<std::vector<int>> layouts;
-        sequence(run, order, lbit, seq, layouts, RS);
-        // load layout: register bits RS, thread bits the other local bits ascending
+        std::vector<int> seq, RS, THR;
+        std::vector<Change> changes;
+        sequence(run, order, lbit, seq, changes, RS, THR);
+        // load layout: register bits RS, thread bits THR
         auto reg_offsets = [&](const std::vector<int> &rs, uint64_t *out) {
             for (int p = 0; p < (1 << RB); p++) {
                 uint64_t g = 0;
@@ -516,19 +639,29 @@ struct Planner {
             }
         };
         reg_offsets(RS, sw.desc.gr);
-        {
-            uint32_t used = 0;
-            for (int b : RS) used |= 1u << b;
-            int t = 0;
-            for (int b = 0; b < K; b++)
-                if (!((used >> b) & 1)) sw.desc.tin[t++] = (int8_t)b;
-        }
+        for (int t = 0; t < LOGW; t++) sw.desc.tin[t] = (int8_t)THR[t];
+        // fast sweeps: uncontrolled H ops leave their 1/sqrt(2) to the next
+        // uncontrolled RX/RY/U/RZ (whose coefficients take the product) or,
+        // at the end of the sweep, to the last of them (a scaled H)
+        std::vector<size_t> pend_h;
+        auto factor = [](size_t d) {
+            return (d & 1) ? std::ldexp(0.70710678118654752440, -(int)(d / 2))
+                           : std::ldexp(1.0, -(int)(d / 2));
+        };
         size_t li_next = 0;
         for (int gi : seq) {
             if (gi < 0) {
-                const std::vector<int> &nx = layouts[li_next++];
-                if (nx != RS) emit_layout(RS, nx);
-                RS = nx;
+                const Change &ch = changes[li_next++];
+                if (ch.lds) {
+                    if (ch.rs != RS || ch.thr != THR) emit_layout(RS, ch.rs, ch.thr);
+                } else {
+                    for (auto [j, tb] : ch.sw) {
+                        emit_swap(j, tb, RS[j], THR[tb]);
+                        std::swap(RS[j], THR[tb]);
+                    }
+                }
+                RS = ch.rs;
+                THR = ch.thr;
                 continue;
             }
             const SG &g = run[gi];
@@ -536,6 +669,13 @@ struct Planner {
             memset(&op, 0, sizeof op);
             op.ak = g.k.ak;
             for (int j = 0; j < 8; j++) op.c[j] = g.k.c[j];
+            const bool uncontrolled = !g.ones && !g.zeros;
+            if (fast && uncontrolled && !pend_h.empty() &&
+                (g.k.ak == AK_RX || g.k.ak == AK_RY || g.k.ak == AK_U || g.k.ak == AK_RZ)) {
+                const double f = factor(pend_h.size());
+                for (int j = 0; j < 8; j++) op.c[j] *= f;
+                pend_h.clear();
+            }
             auto slot_of = [&](int b) {
                 for (int j = 0; j < RB; j++)
                     if (RS[j] == b) return j;
@@ -567,6 +707,10 @@ struct Planner {
                 return;
             }
             op.grp = TG_GATE | kbit;
+            if (fast && g.k.ak == AK_H) {
+                if (uncontrolled) pend_h.push_back(prog.size());
+                else op.grp = (op.grp & ~TG_H) | TG_HS;  // applies its own factor
+            }
             if (g.diag) {
                 op.type = TOP_DIAG;
                 const int b = lbit[g.target];
@@ -596,8 +740,14 @@ struct Planner {
             }
             prog.push_back(op);
         }
-        // store layout: the last one, unless it holds a low bit in registers;
-        // then those register bits are swapped for unused high bits
+        if (!pend_h.empty()) {
+            TileOp &h = prog[pend_h.back()];
+            h.grp = (h.grp & ~TG_H) | TG_HS;
+            h.c[0] = factor(pend_h.size());
+        }
+        // store layout: the last one, unless it holds a low bit in registers
+        // (then those register bits are swapped for unused high bits) or the
+        // low lane bits do not hold the low local bits in order
         {
             std::vector<int> st = RS;
             uint32_t used = 0;
@@ -613,7 +763,9 @@ struct Planner {
                 st[j] = nb;
                 used |= 1u << nb;
             }
-            if (st != RS) emit_layout(RS, st);
+            bool lanes_ok = true;
+            for (int i = 0; i < klo && i < LOGW; i++) lanes_ok &= THR[i] == i;
+            if (st != RS || !lanes_ok) emit_layout(RS, st, thread_layout(st, nullptr, 0));
             reg_offsets(st, sw.desc.gs);
         }
         sw.op_count = prog.size() - sw.op_begin;
@@ -706,6 +858,8 @@ int run_fused(spz_state_s *s, const std::vector<SG> &run, int K, bool strict)
     pl.klo = std::min(env_int("SPZ_TILE_KLO", 4), K);
     pl.improve = env_int("SPZ_TILE_SEARCH", 16);
     pl.strict = strict;
+    pl.fast = !strict;
+    pl.swaps = env_int("SPZ_TILE_SWAPS", 1) != 0;
     pl.plan(run);
     if (pl.bad) return fail(SPZ_ERR_ARG, "tile planner produced a gate outside the register layout");
     std::lock_guard<std::mutex> lk(g_slot_mu);
@@ -725,7 +879,7 @@ int run_fused(spz_state_s *s, const std::vector<SG> &run, int K, bool strict)
     } release{slot};
     for (const Sweep &sw : pl.sweeps) {
         if (sw.tile) {
-            SPZ_TRY(launch_tile(s, K, sw.desc, d_prog + sw.op_begin));
+            SPZ_TRY(launch_tile(s, K, sw.desc, d_prog + sw.op_begin, pl.fast));
         } else {
             const SG &g = run[sw.gate];
             if (g.diag) SPZ_TRY(launch_diag(s, g.k, g.target, g.ones, g.zeros));
@@ -933,6 +1087,8 @@ int plan_info(int n, int prec, const spz_op *ops, uint64_t n_ops, const spz_exec
         pl.klo = std::min(env_int("SPZ_TILE_KLO", 4), K);
         pl.improve = env_int("SPZ_TILE_SEARCH", 16);
         pl.strict = mode == SPZ_EXEC_STRICT;
+        pl.fast = !pl.strict;
+        pl.swaps = env_int("SPZ_TILE_SWAPS", 1) != 0;
         pl.plan(run);
         if (pl.bad) return fail(SPZ_ERR_ARG, "tile planner produced a gate outside the register layout");
         for (const Sweep &sw : pl.sweeps) {
@@ -942,8 +1098,9 @@ int plan_info(int n, int prec, const spz_op *ops, uint64_t n_ops, const spz_exec
             }
             out->sweeps++;
             for (size_t o = sw.op_begin; o < sw.op_begin + sw.op_count; o++) {
-                if (pl.prog[o].type == TOP_LAYOUT) out->layouts++;
-                else out->tile_gates++;
+                if (pl.prog[o].type != TOP_LAYOUT) out->tile_gates++;
+                else if (pl.prog[o].grp & TG_LAYOUT) out->layouts++;
+                else out->lane_swaps++;
             }
         }
         i = j;

@@ -12,7 +12,9 @@
 // a gate whose target is a register bit is a pure in-register pair update,
 // with controls and diagonal phases evaluated from each amplitude's local
 // index (or, for qubits outside S, once per tile).  Layout changes go through
-// LDS.  The layouts a sweep loads in and stores from keep the low local bits
+// LDS, or, when a register bit only trades places with lane bit 4 or 5, stay
+// inside the wavefront (v_permlane16/32_swap).  The layouts a sweep loads in
+// and stores from keep the low local bits
 // (the lowest qubits) on the low lane bits, so lanes of a wavefront read and
 // write contiguous runs; the register bits at load and store are chosen per
 // sweep, so that a sweep whose first or last layout meets that rule needs no
@@ -39,7 +41,14 @@ enum TileGroup : uint32_t {
     TG_H = 1u << 8, TG_X = 1u << 9, TG_Y = 1u << 10, TG_RX = 1u << 11, TG_RY = 1u << 12,
     TG_U = 1u << 13, TG_NEG = 1u << 15, TG_PHASE = 1u << 16,
     TG_RZ = 1u << 17,
+    // in-wavefront layout changes: register-index bit `sel` trades places
+    // with lane bit 4 (v_permlane16_swap) or lane bit 5 (v_permlane32_swap)
+    TG_SW16 = 1u << 18, TG_SW32 = 1u << 19,
+    // fast sweeps (AUTO): an H whose 1/sqrt(2) is applied in place, scaled by
+    // c[0]; plain TG_H then adds and subtracts only (the factor is carried)
+    TG_HS = 1u << 20,
 };
+constexpr uint32_t TG_ANYLAYOUT = TG_LAYOUT | TG_SW16 | TG_SW32;
 
 // LDS bank swizzle for 8-byte elements of a tile: XOR the low 4 bits of the
 // local index with two higher nibbles so that power-of-two strides across
@@ -74,7 +83,9 @@ struct TileOp {
     int32_t dpos;
     uint32_t pm;        // bit p: register p meets the controls on register-index bits
     uint32_t pad;
-    uint32_t mlo, mlz;  // controls on local bits held by the lane part (per lane)
+    uint32_t mlo, mlz;  // controls on local bits held by the lane part (per lane);
+                        // TG_SW16/TG_SW32: the local bit leaving the lane part
+                        // (mlo, as a mask) and the one entering it (mlz)
     uint32_t grp;       // TileGroup bits
     uint64_t eo, em;    // controls outside S: the op runs on a tile iff (base & em) == eo
                         // (a layout change has em = 0, eo = 1 and never runs as a gate)
@@ -99,6 +110,8 @@ struct TileDesc {
     int32_t pad;
 };
 
-int launch_tile(spz_state_s *s, int K, const TileDesc &d, const TileOp *d_ops);
+// fast: the AUTO arithmetic (FMA-form RX/RY, H with the factor carried; see
+// the kernel); otherwise every gate is the reference's operation sequence.
+int launch_tile(spz_state_s *s, int K, const TileDesc &d, const TileOp *d_ops, bool fast);
 
 }  // namespace spz

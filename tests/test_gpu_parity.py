@@ -445,3 +445,42 @@ def test_back_to_back_executes_without_sync(spz, orc):
         run_ops(s, ops, mode=N.EXEC_STRICT)
     for k, (s, (_, _, ref)) in enumerate(zip(states, cases)):
         assert_match(s, ref, exact=True, what=f"back-to-back circuit {k}")
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("tile_qubits", [10, 12])
+def test_lane_swaps_and_fast_sweeps_vs_oracle(spz, orc, tile_qubits):
+    """Sweeps whose layout changes include in-wavefront lane swaps (register
+    bit <-> lane bit 4 or 5, v_permlane16/32_swap), and the AUTO arithmetic:
+    runs of H with nothing to absorb their factor before the sweep ends, H
+    followed by RX / RY / U / RZ that absorb it, controlled H (which applies
+    its own factor).  STRICT (swaps, reference arithmetic) is bit-identical to
+    the oracle; AUTO within 1e-12.  The plan must contain lane swaps."""
+    print(f"case lane swaps / fast sweeps K={tile_qubits}", flush=True)
+    from spinoza_amd import _native as N
+    from spinoza_amd.executor import plan_stats, run_ops
+    from spinoza_amd.workloads import op
+    n = 15
+    rng = np.random.default_rng(91 + tile_qubits)
+    ops = []
+    for layer in range(12):
+        for q in range(n):
+            k = [N.H, N.H, N.RX, N.RY, N.U, N.RZ, N.Y, N.X][int(rng.integers(8))]
+            ops.append(op(k, q, theta=float(rng.uniform(0, 6.28)), phi=float(rng.uniform(0, 6.28)),
+                          lam=float(rng.uniform(0, 6.28))))
+        for _ in range(4):
+            c, t = (int(x) for x in rng.choice(n, 2, replace=False))
+            ops.append(op(N.H if layer % 3 == 0 else N.X, t, ones=1 << c))
+        for q in range(0, n, 3):  # runs of bare H
+            ops.append(op(N.H, q))
+            ops.append(op(N.H, (q + 1) % n))
+    st = plan_stats(n, ops, mode=N.EXEC_AUTO, tile_qubits=tile_qubits)
+    ss = plan_stats(n, ops, mode=N.EXEC_STRICT, tile_qubits=tile_qubits)
+    assert st["lane_swaps"] > 0 and ss["lane_swaps"] > 0, (st, ss)
+    base = orc.random_state(12, n)
+    ref = base.copy()
+    orc.execute(ref, ops)
+    for mode in (N.EXEC_STRICT, N.EXEC_AUTO):
+        s = to_gpu(spz, base)
+        run_ops(s, ops, mode=mode, tile_qubits=tile_qubits)
+        assert_match(s, ref, exact=(mode == N.EXEC_STRICT), what=f"swaps/fast mode={mode}")
