@@ -921,42 +921,63 @@ int run_dense5(spz_state_s *s, const std::vector<SG> &run)
         std::vector<int> gates;
         int single;
     };
+    // Blocks are formed greedily over the remaining gates in program order:
+    // a block takes a gate whose qubits keep it within 5 qubits and that
+    // commutes with every gate the block has passed over (the tile planner's
+    // rule: a passed gate blocks later gates on its non-diagonal qubits, and
+    // non-diagonal action on its diagonal qubits), so a block gathers the
+    // gates of a 5-qubit region across layers rather than only a consecutive
+    // run.  The taken gates keep their order inside the block; a block of one
+    // gate, or a gate on more than 5 qubits, runs on the plain path.
     std::vector<Blk> blocks;
-    uint64_t Q = 0;
-    std::vector<int> cur;
     const uint64_t all = s->n >= 64 ? ~0ull : ((1ull << s->n) - 1);
-    auto flush = [&]() {
-        if (cur.empty()) return;
+    std::vector<int> rem(run.size());
+    for (size_t i = 0; i < run.size(); i++) rem[i] = (int)i;
+    while (!rem.empty()) {
+        const SG &g0 = run[rem[0]];
+        if (__builtin_popcountll((g0.nd | g0.dm) & all) > 5) {
+            Blk b;
+            b.single = rem[0];
+            blocks.push_back(b);
+            rem.erase(rem.begin());
+            continue;
+        }
+        uint64_t Q = 0, bnd = 0, bd = 0;
+        std::vector<int> take, left;
+        for (size_t r = 0; r < rem.size(); r++) {
+            const int idx = rem[r];
+            const SG &g = run[idx];
+            const uint64_t qs = (g.nd | g.dm) & all;
+            const bool ok = !(g.nd & (bnd | bd)) && !(g.dm & bnd) &&
+                            __builtin_popcountll(Q | qs) <= 5;
+            if (ok) {
+                take.push_back(idx);
+                Q |= qs;
+            } else {
+                left.push_back(idx);
+                bnd |= g.nd;
+                bd |= g.dm;
+                if (((bnd | bd) & all) == all) {  // nothing further can pass
+                    left.insert(left.end(), rem.begin() + (long)r + 1, rem.end());
+                    break;
+                }
+            }
+        }
+        rem.swap(left);
         Blk b;
         b.single = -1;
-        if (cur.size() == 1) {
-            b.single = cur[0];
+        if (take.size() == 1) {
+            b.single = take[0];
         } else {
             int j = 0;
             for (int q = 0; q < s->n && j < 5; q++)
                 if ((Q >> q) & 1) b.q[j++] = q;
             for (int q = 0; q < s->n && j < 5; q++)
                 if (!((Q >> q) & 1)) { b.q[j++] = q; Q |= 1ull << q; }
-            b.gates = cur;
+            b.gates = take;
         }
         blocks.push_back(b);
-        cur.clear();
-        Q = 0;
-    };
-    for (size_t i = 0; i < run.size(); i++) {
-        const uint64_t qs = (run[i].nd | run[i].dm) & all;
-        if (__builtin_popcountll(qs) > 5) {
-            flush();
-            Blk b;
-            b.single = (int)i;
-            blocks.push_back(b);
-            continue;
-        }
-        if (__builtin_popcountll(Q | qs) > 5) flush();
-        Q |= qs;
-        cur.push_back((int)i);
     }
-    flush();
     // block matrices: U = G_last ... G_first, embedded in the 5-qubit block
     std::vector<double> f64;
     std::vector<float> f32;
