@@ -77,6 +77,11 @@ def parse():
                    help="layers timed through the dense 5-qubit blocks at 32q/34q")
     p.add_argument("--no-batch", action="store_true", help="skip the batched adjoint leg")
     p.add_argument("--batch-count", type=int, default=1024)
+    p.add_argument("--batch-group-qubits", type=int, default=0,
+                   help="circuits above this many qubits run on groups of workgroups "
+                        "(0: the library default, < 0: never)")
+    p.add_argument("--batch-wg-max", type=int, default=0,
+                   help="persistent-kernel switch point (0: autotuned)")
     p.add_argument("--no-qft", action="store_true", help="skip the 20q QFT leg")
     return p.parse_args()
 
@@ -239,7 +244,7 @@ def fusion_leg(spz, N, layers, n64, n32, n_prec, dense_layers, seed=3):
     return out
 
 
-def batch_leg(spz, N, count, seed=5):
+def batch_leg(spz, N, count, seed=5, group_qubits=0, wg_max=0):
     """`count` hardware-efficient-ansatz circuits, n ~ U{10..20}, depth ~
     U{2..12} (SplitMix64 seed 5), Ising energy (hx = 1): one batched call
     after autotuning the per-workgroup switch point, against looping the
@@ -257,9 +262,10 @@ def batch_leg(spz, N, count, seed=5):
         ops, _, where = hea(n, depth, seed=seed * 100003 + i)
         circuits.append((n, ops))
         nparams += len(where)
-    wg = batch_autotune(circuits)
-    batch_adjoint(circuits, 1.0)  # warm (graph capture, buffers)
-    dt, (energies, grads, st) = timed(lambda: batch_adjoint(circuits, 1.0), sync)
+    wg = wg_max or batch_autotune(circuits)
+    run = lambda: batch_adjoint(circuits, 1.0, wg_max_qubits=wg, group_qubits=group_qubits)
+    run()  # warm (graph capture, buffers)
+    dt, (energies, grads, st) = timed(run, sync)
     states = {}
     for n, _ in circuits:
         if n not in states:
@@ -279,7 +285,7 @@ def batch_leg(spz, N, count, seed=5):
         err = max(err, abs(e - e1), float(np.max(np.abs(g - g1))))
     del states
     return {"circuits": count, "params": nparams, "wg_max_qubits": wg,
-            "n_small": st["n_small"], "n_large": st["n_large"],
+            "n_small": st["n_small"], "n_large": st["n_large"], "n_group": st["n_group"],
             "total_ms": round(dt * 1e3, 2), "p50_ms": round(st["p50_ms"], 4),
             "p99_ms": round(st["p99_ms"], 4), "grads_per_s": round(nparams / dt, 1),
             "loop_single_ms": round(dt_loop * 1e3, 2),
@@ -448,7 +454,8 @@ def main():
 
     if not a.no_batch:
         log(f"batched adjoint leg: {a.batch_count} circuits")
-        result["batch"] = batch_leg(spz, N, a.batch_count)
+        result["batch"] = batch_leg(spz, N, a.batch_count, group_qubits=a.batch_group_qubits,
+                                    wg_max=a.batch_wg_max)
 
     qft_ctx = None
     if not a.no_qft:

@@ -267,11 +267,15 @@ typedef struct spz_batch_circuit {
 } spz_batch_circuit;
 
 typedef struct spz_batch_opts {
-    int32_t wg_max_qubits;  /* circuits with n <= this run one per workgroup in
-                               the persistent kernel; 0 = autotuned value
-                               (or 14 before spz_batch_autotune ran)        */
+    int32_t wg_max_qubits;  /* circuits with n <= this run in the persistent
+                               kernel; 0 = autotuned value (or 14 before
+                               spz_batch_autotune ran)                      */
     int32_t use_graph;      /* capture the persistent launch in a hipGraph  */
-    int32_t reserved[6];
+    int32_t group_qubits;   /* circuits in the persistent kernel with more
+                               qubits run on 2^(n - group_qubits) workgroups
+                               of one XCD; 0 = 12, < 0 = never (values
+                               below 8 count as 8)                          */
+    int32_t reserved[5];
 } spz_batch_opts;
 
 typedef struct spz_batch_stats {
@@ -279,7 +283,7 @@ typedef struct spz_batch_stats {
     double p50_ms, p99_ms;  /* per-circuit processing time                  */
     uint64_t n_small, n_large;
     int32_t wg_max_qubits;
-    int32_t reserved;
+    int32_t n_group;        /* circuits run on a group of workgroups        */
 } spz_batch_stats;
 
 /* Energy and adjoint gradient (as spz_adjoint_grad, no checkpoints) of every

@@ -66,7 +66,7 @@ class BatchCircuit_t(ctypes.Structure):
 
 class BatchOpts_t(ctypes.Structure):
     _fields_ = [("wg_max_qubits", ctypes.c_int32), ("use_graph", ctypes.c_int32),
-                ("reserved", ctypes.c_int32 * 6)]
+                ("group_qubits", ctypes.c_int32), ("reserved", ctypes.c_int32 * 5)]
 
 
 class PlanInfo_t(ctypes.Structure):
@@ -79,7 +79,7 @@ class BatchStats_t(ctypes.Structure):
     _fields_ = [("total_ms", ctypes.c_double), ("p50_ms", ctypes.c_double),
                 ("p99_ms", ctypes.c_double), ("n_small", ctypes.c_uint64),
                 ("n_large", ctypes.c_uint64), ("wg_max_qubits", ctypes.c_int32),
-                ("reserved", ctypes.c_int32)]
+                ("n_group", ctypes.c_int32)]
 
 
 # name -> (restype, argtypes) for every exported entry point

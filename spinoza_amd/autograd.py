@@ -115,10 +115,13 @@ class IsingEnergy:
         return cls._fn.apply(thetas, builder, n, hx, ckpt_every, precision)
 
 
-def batch_adjoint(circuits, hx=1.0, wg_max_qubits=0, use_graph=True):
+def batch_adjoint(circuits, hx=1.0, wg_max_qubits=0, use_graph=True, group_qubits=0):
     """Energies and adjoint gradients of many independent circuits.
 
     `circuits` is a list of (n, ops); every circuit starts in |0...0>.
+    Circuits up to `wg_max_qubits` run in the persistent kernel, those above
+    `group_qubits` of them on a group of workgroups (0: the defaults; a
+    negative group_qubits keeps every circuit in one workgroup).
     Returns (energies, [grad_per_op arrays], stats dict)."""
     all_ops, table, off, goff = [], [], 0, 0
     for n, ops in circuits:
@@ -135,13 +138,15 @@ def batch_adjoint(circuits, hx=1.0, wg_max_qubits=0, use_graph=True):
     opts = N.BatchOpts_t()
     opts.wg_max_qubits = int(wg_max_qubits)
     opts.use_graph = int(bool(use_graph))
+    opts.group_qubits = int(group_qubits)
     st = N.BatchStats_t()
     N.check(N.lib().spz_batch_adjoint(cb, len(table), arr, float(hx), N.dptr(energies),
                                       N.dptr(grads), ctypes.byref(opts), ctypes.byref(st)))
     del keep
     out = [grads[g:g + k].copy() for (n, k, o, g) in table]
     stats = {"total_ms": st.total_ms, "p50_ms": st.p50_ms, "p99_ms": st.p99_ms,
-             "n_small": st.n_small, "n_large": st.n_large, "wg_max_qubits": st.wg_max_qubits}
+             "n_small": st.n_small, "n_large": st.n_large, "n_group": st.n_group,
+             "wg_max_qubits": st.wg_max_qubits}
     return energies[:len(table)], out, stats
 
 

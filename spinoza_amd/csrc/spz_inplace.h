@@ -108,28 +108,29 @@ __device__ __forceinline__ void ip_ry(double sn, double cs, double &a, double &b
 //   a = fma(g, cc, fma(k, m, -l n))          b = fma(g, dd, fma(k, n, l m))
 //   c = fma(q, cc, fma(-r, dd, fma(s, m, -t n)))
 //   d = fma(q, dd, fma(r, cc, fma(s, n, t m)))
-// c and d are formed first (they need all four inputs), then a and b in place.
+// The inner terms (all that read m and n) are formed first into four
+// temporaries; then c and d overwrite m and n, and a and b overwrite cc and
+// dd after their last use: 14 instructions, no register copies, and every
+// expression keeps the reference's operation order.
 __device__ __forceinline__ void ip_u(const double *k, double &a, double &b, double &c, double &d)
 {
-    double t0, t1, x;
+    double x1, x2, p1, p2;
     asm volatile(
-        "v_mul_f64 %0, -%13, %6\n\t"     // -t n
-        "v_fma_f64 %0, %12, %5, %0\n\t"  // fma(s, m, .)
-        "v_fma_f64 %0, -%11, %4, %0\n\t" // fma(-r, dd, .)
-        "v_fma_f64 %0, %10, %3, %0\n\t"  // c' = fma(q, cc, .)
-        "v_mul_f64 %1, %13, %5\n\t"      // t m
-        "v_fma_f64 %1, %12, %6, %1\n\t"  // fma(s, n, .)
-        "v_fma_f64 %1, %11, %3, %1\n\t"  // fma(r, cc, .)
-        "v_fma_f64 %1, %10, %4, %1\n\t"  // d' = fma(q, dd, .)
-        "v_mul_f64 %2, -%9, %6\n\t"      // -l n
-        "v_fma_f64 %2, %8, %5, %2\n\t"   // fma(k, m, .)
-        "v_fma_f64 %3, %7, %3, %2\n\t"   // a' = fma(g, cc, .)
-        "v_mul_f64 %2, %9, %5\n\t"       // l m
-        "v_fma_f64 %2, %8, %6, %2\n\t"   // fma(k, n, .)
-        "v_fma_f64 %4, %7, %4, %2\n\t"   // b' = fma(g, dd, .)
-        "v_mov_b64 %5, %0\n\t"
-        "v_mov_b64 %6, %1"
-        : "=&v"(t0), "=&v"(t1), "=&v"(x), "+v"(a), "+v"(b), "+v"(c), "+v"(d)
+        "v_mul_f64 %0, -%14, %7\n\t"     // -t n
+        "v_fma_f64 %0, %13, %6, %0\n\t"  // x1 = fma(s, m, -t n)
+        "v_mul_f64 %1, %14, %6\n\t"      // t m
+        "v_fma_f64 %1, %13, %7, %1\n\t"  // x2 = fma(s, n, t m)
+        "v_mul_f64 %2, -%10, %7\n\t"     // -l n
+        "v_fma_f64 %2, %9, %6, %2\n\t"   // p1 = fma(k, m, -l n)
+        "v_mul_f64 %3, %10, %6\n\t"      // l m
+        "v_fma_f64 %3, %9, %7, %3\n\t"   // p2 = fma(k, n, l m)
+        "v_fma_f64 %6, -%12, %5, %0\n\t" // fma(-r, dd, x1)
+        "v_fma_f64 %6, %11, %4, %6\n\t"  // c' = fma(q, cc, .)
+        "v_fma_f64 %7, %12, %4, %1\n\t"  // fma(r, cc, x2)
+        "v_fma_f64 %7, %11, %5, %7\n\t"  // d' = fma(q, dd, .)
+        "v_fma_f64 %4, %8, %4, %2\n\t"   // a' = fma(g, cc, p1)
+        "v_fma_f64 %5, %8, %5, %3"         // b' = fma(g, dd, p2)
+        : "=&v"(x1), "=&v"(x2), "=&v"(p1), "=&v"(p2), "+v"(a), "+v"(b), "+v"(c), "+v"(d)
         : "s"(k[0]), "s"(k[2]), "s"(k[3]), "s"(k[4]), "s"(k[5]), "s"(k[6]), "s"(k[7]));
 }
 
