@@ -406,11 +406,22 @@ def main():
 
     ms_step = (t1 - t0) * 1e3 / a.steps
     value = len(ops) * a.steps / (t1 - t0)
-    traffic = None
+    # HBM bytes per launch from the committed rocprofv3 FETCH_SIZE/WRITE_SIZE
+    # summary of this kernel on this workload; kept only while the profiled
+    # launch time is within 15% of the live one (a changed kernel makes the
+    # summary stale, and the line then says so)
+    traffic, traffic_src = None, None
     tpath = os.path.join(ROOT, "profiles", f"traffic_{dom}_{n}q.json")
     if os.path.exists(tpath):
         with open(tpath) as f:
-            traffic = json.load(f).get("hbm_bytes_per_launch")
+            tj = json.load(f)
+        live_us = ms / max(cnt, 1) * 1e3
+        prof_us = tj.get("avg_launch_us") or 0.0
+        fresh = prof_us > 0 and abs(prof_us - live_us) <= 0.15 * live_us
+        traffic = tj.get("hbm_bytes_per_launch") if fresh else None
+        traffic_src = {"file": f"profiles/traffic_{dom}_{n}q.json", "kernel": tj.get("kernel"),
+                       "profiled_launch_us": prof_us, "live_launch_us": round(live_us, 1),
+                       "used": bool(fresh)}
 
     result = {
         "metric": "30q f64 gates/s + % HBM roofline; 24q adjoint grads/s vs param-shift",
@@ -431,6 +442,7 @@ def main():
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(achieved / 1e9, 1),
                      "peak": HBM_PEAK / 1e9, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK, 4), "traffic": traffic,
+                     "traffic_source": traffic_src,
                      "launches": cnt, "avg_launch_ms": round(ms / max(cnt, 1), 4),
                      "alg_bytes_per_launch": by / max(cnt, 1)},
         "kernel_time_ms": {k: round(v[1], 2) for k, v in prof.items() if v[0]},
