@@ -253,7 +253,9 @@ typedef struct spz_plan_info {
     uint64_t layouts;       /* register-layout changes inside sweeps             */
     uint64_t plain_gates;   /* gates left to one-kernel-per-gate passes          */
     uint64_t lane_swaps;    /* in-wavefront layout changes (v_permlane*_swap)   */
-    uint64_t reserved[3];
+    uint64_t frame_gates;   /* X, Y, Z absorbed into a sweep's Pauli frame (AUTO;
+                               counted in tile_gates too)                        */
+    uint64_t reserved[2];
 } spz_plan_info;
 int spz_plan_stats(int n, int precision, const spz_op *ops, uint64_t n_ops,
                    const spz_exec_opts *opts, spz_plan_info *out);

@@ -72,7 +72,8 @@ class BatchOpts_t(ctypes.Structure):
 class PlanInfo_t(ctypes.Structure):
     _fields_ = [("sweeps", ctypes.c_uint64), ("tile_gates", ctypes.c_uint64),
                 ("layouts", ctypes.c_uint64), ("plain_gates", ctypes.c_uint64),
-                ("lane_swaps", ctypes.c_uint64), ("reserved", ctypes.c_uint64 * 3)]
+                ("lane_swaps", ctypes.c_uint64), ("frame_gates", ctypes.c_uint64),
+                ("reserved", ctypes.c_uint64 * 2)]
 
 
 class BatchStats_t(ctypes.Structure):

@@ -141,8 +141,8 @@ struct Sweep {
     size_t op_begin, op_count; // range in the program buffer
 };
 
-// One-hot TileGroup bit of an arithmetic kind (0: not a tile kind; the gates
-// lowered by lower() never carry AK_MAT).
+// One-hot TileGroup bit of an arithmetic kind (0: not a tile kind).  AK_MAT
+// comes only from a fast sweep's frame conjugation (lower() never makes it).
 uint32_t kind_bit(int ak)
 {
     switch (ak) {
@@ -155,6 +155,7 @@ uint32_t kind_bit(int ak)
     case AK_NEG: return TG_NEG;
     case AK_PHASE: return TG_PHASE;
     case AK_RZ: return TG_RZ;
+    case AK_MAT: return TG_MAT;
     default: return 0;
     }
 }
@@ -169,6 +170,74 @@ struct Change {
     std::vector<std::pair<int, int>> sw;    // (register slot, lane bit), in order
 };
 
+void gate_matrix2(const Coef &k, double m[8]);
+
+// Pauli frame of a fast sweep: the registers hold phi, and the state is
+// psi = i^w X^x Z^z phi (X^x flips the qubits of x, Z^z negates amplitudes of
+// odd parity on z).  Uncontrolled X, Y and Z change only the frame; H, CX and
+// CZ run unchanged and carry the frame through (Clifford conjugation); every
+// other gate runs conjugated by the frame, which leaves the frame as it is.
+// The store applies it (see TileDesc).
+struct Frame {
+    uint64_t x = 0, z = 0;
+    int w = 0;
+};
+
+// Coefficients of Z^b X^a G X^a Z^b: gate G seen through a frame with X^a Z^b
+// on its target.  Keeps the kind where one covers the result (RX, RY, RZ, U
+// under Z; a diagonal stays diagonal), else a general matrix (AK_MAT).
+void conj_coef(Coef &k, bool a, bool b)
+{
+    if (!a && !b) return;
+    switch (k.ak) {
+    case AK_RX:  // [[c, -is], [-is, c]]: X keeps it, Z negates s
+        if (b) k.c[1] = -k.c[1];
+        return;
+    case AK_RY:  // [[c, -s], [s, c]]: X and Z each negate s
+        if (a != b) k.c[0] = -k.c[0];
+        return;
+    case AK_RZ:
+        if (a) {
+            std::swap(k.c[0], k.c[2]);
+            std::swap(k.c[1], k.c[3]);
+        }
+        return;
+    case AK_PHASE:  // X diag(1, p) X = diag(p, 1)
+        if (a) {
+            const double pr = k.c[0], pi = k.c[1];
+            memset(k.c, 0, sizeof k.c);
+            k.ak = AK_RZ;
+            k.c[0] = pr; k.c[1] = pi; k.c[2] = 1.0;
+        }
+        return;
+    case AK_NEG:  // X Z X = diag(-1, 1)
+        if (a) {
+            memset(k.c, 0, sizeof k.c);
+            k.ak = AK_RZ;
+            k.c[0] = -1.0; k.c[2] = 1.0;
+        }
+        return;
+    case AK_U:  // Z negates the off-diagonal entries; X needs a complex u00
+        if (!a) {
+            for (int i = 2; i < 6; i++) k.c[i] = -k.c[i];
+            return;
+        }
+        break;
+    default:
+        break;
+    }
+    double m[8];
+    gate_matrix2(k, m);
+    if (a) {  // X G X: rows and columns swapped
+        const double r[8] = {m[6], m[7], m[4], m[5], m[2], m[3], m[0], m[1]};
+        memcpy(m, r, sizeof r);
+    }
+    if (b)
+        for (int i = 2; i < 6; i++) m[i] = -m[i];
+    k.ak = AK_MAT;
+    memcpy(k.c, m, sizeof m);
+}
+
 struct Planner {
     int n, K, LOGW, RB, klo;
     int improve = 0;  // local-search passes over the tile set (AUTO only)
@@ -176,8 +245,18 @@ struct Planner {
     bool fast = false;   // AUTO arithmetic (see launch_tile): H factors carried
     bool swaps = true;   // in-wavefront layout changes when they reach the set
     bool bad = false;
+    uint64_t frame_gates = 0;  // gates absorbed into Pauli frames
     std::vector<TileOp> prog;
     std::vector<Sweep> sweeps;
+
+    // A gate that a fast sweep absorbs into its frame needs no register slot.
+    bool absorbed(const SG &g) const
+    {
+        return fast && !g.ones && !g.zeros &&
+               (g.k.ak == AK_X || g.k.ak == AK_Y || g.k.ak == AK_NEG);
+    }
+    // Gates that run only with their target on a register bit.
+    bool needs_reg(const SG &g) const { return !g.diag && !absorbed(g); }
 
     // Choose the tile qubits and the gates of one sweep from `rem` (indices
     // into `run`, in program order).  Returns the chosen gate indices.
@@ -397,7 +476,7 @@ struct Planner {
             for (int a = 0; a < m; a++) {
                 if (done[a] || indeg[a]) continue;
                 const SG &g = run[order[a]];
-                if (!g.diag && !((rs >> lbit[g.target]) & 1)) continue;
+                if (needs_reg(g) && !((rs >> lbit[g.target]) & 1)) continue;
                 done[a] = 1;
                 cnt++;
                 for (int b : succ[a]) indeg[b]--;
@@ -500,7 +579,7 @@ struct Planner {
             for (int a = 0; a < m; a++) {
                 if (done && (*done)[a]) continue;
                 const SG &g = run[order[a]];
-                if (!g.diag) score[lbit[g.target]] += indeg[a] == 0 ? 1000 : 1;
+                if (needs_reg(g)) score[lbit[g.target]] += indeg[a] == 0 ? 1000 : 1;
             }
             return score;
         };
@@ -587,7 +666,7 @@ struct Planner {
                 for (int a = 0; a < m; a++) {
                     if (done[a] || indeg[a]) continue;
                     const SG &g = run[order[a]];
-                    if (!g.diag && !in_rs(lbit[g.target])) continue;
+                    if (needs_reg(g) && !in_rs(lbit[g.target])) continue;
                     seq.push_back(order[a]);
                     done[a] = 1;
                     left--;
@@ -648,6 +727,7 @@ struct Planner {
             return (d & 1) ? std::ldexp(0.70710678118654752440, -(int)(d / 2))
                            : std::ldexp(1.0, -(int)(d / 2));
         };
+        Frame fr;
         size_t li_next = 0;
         for (int gi : seq) {
             if (gi < 0) {
@@ -664,14 +744,53 @@ struct Planner {
                 THR = ch.thr;
                 continue;
             }
-            const SG &g = run[gi];
+            SG g = run[gi];
+            // Pauli frame (fast sweeps): Paulis are absorbed, H / CX / CZ carry
+            // the frame (updated below, once the op is out), other gates are
+            // conjugated
+            enum { KEEP, PUSH_H, PUSH_CX, PUSH_CZ } push = KEEP;
+            const uint64_t tb = 1ull << g.target;
+            const bool fa = (fr.x & tb) != 0, fb = (fr.z & tb) != 0;
+            if (fast) {
+                const bool unc = !g.ones && !g.zeros;
+                const bool one_ctrl = !g.zeros && g.ones && !(g.ones & (g.ones - 1));
+                if (absorbed(g)) {
+                    if (g.k.ak == AK_X) {
+                        fr.x ^= tb;
+                    } else if (g.k.ak == AK_Y) {  // Y = i X Z
+                        fr.w += 1 + 2 * fa;
+                        fr.x ^= tb;
+                        fr.z ^= tb;
+                    } else {  // Z
+                        fr.w += 2 * fa;
+                        fr.z ^= tb;
+                    }
+                    frame_gates++;
+                    continue;
+                }
+                if (unc && g.k.ak == AK_H) {
+                    push = PUSH_H;
+                } else if (one_ctrl && g.k.ak == AK_X) {
+                    push = PUSH_CX;
+                } else if (one_ctrl && g.k.ak == AK_NEG) {
+                    push = PUSH_CZ;
+                } else {
+                    // an X in the frame on a control inverts the condition
+                    const uint64_t fl = (g.ones | g.zeros) & fr.x;
+                    const uint64_t o2 = (g.ones & ~fl) | (g.zeros & fl);
+                    g.zeros = (g.zeros & ~fl) | (g.ones & fl);
+                    g.ones = o2;
+                    conj_coef(g.k, fa, fb);
+                }
+            }
             TileOp op;
             memset(&op, 0, sizeof op);
             op.ak = g.k.ak;
             for (int j = 0; j < 8; j++) op.c[j] = g.k.c[j];
             const bool uncontrolled = !g.ones && !g.zeros;
             if (fast && uncontrolled && !pend_h.empty() &&
-                (g.k.ak == AK_RX || g.k.ak == AK_RY || g.k.ak == AK_U || g.k.ak == AK_RZ)) {
+                (g.k.ak == AK_RX || g.k.ak == AK_RY || g.k.ak == AK_U || g.k.ak == AK_RZ ||
+                 g.k.ak == AK_MAT)) {
                 const double f = factor(pend_h.size());
                 for (int j = 0; j < 8; j++) op.c[j] *= f;
                 pend_h.clear();
@@ -739,6 +858,21 @@ struct Planner {
                 op.grp |= pred ? TG_PAIRP : TG_PAIR;
             }
             prog.push_back(op);
+            if (push == PUSH_H) {  // H X^a Z^b = (-1)^(ab) X^b Z^a H
+                fr.x = (fr.x & ~tb) | (fb ? tb : 0);
+                fr.z = (fr.z & ~tb) | (fa ? tb : 0);
+                fr.w += 2 * (fa && fb);
+            } else if (push == PUSH_CX) {  // X_c -> X_c X_t, Z_t -> Z_c Z_t
+                const uint64_t cb = g.ones;
+                if (fr.x & cb) fr.x ^= tb;
+                if (fb) fr.z ^= cb;
+            } else if (push == PUSH_CZ) {  // X_c -> X_c Z_t, X_t -> Z_c X_t
+                const uint64_t cb = g.ones;
+                const bool xc = (fr.x & cb) != 0;
+                fr.w += 2 * (xc && fa);
+                if (xc) fr.z ^= tb;
+                if (fa) fr.z ^= cb;
+            }
         }
         if (!pend_h.empty()) {
             TileOp &h = prog[pend_h.back()];
@@ -767,6 +901,37 @@ struct Planner {
             for (int i = 0; i < klo && i < LOGW; i++) lanes_ok &= THR[i] == i;
             if (st != RS || !lanes_ok) emit_layout(RS, st, thread_layout(st, nullptr, 0));
             reg_offsets(st, sw.desc.gs);
+            // the frame, split by where the store layout holds each bit
+            if (fr.x & ~smask) {  // frames only flip qubits the sweep targets
+                bad = true;
+                return;
+            }
+            uint32_t xl = 0, zl = 0, regm = 0;
+            for (int q = 0; q < n; q++) {
+                if (lbit[q] < 0) continue;
+                if ((fr.x >> q) & 1) xl |= 1u << lbit[q];
+                if ((fr.z >> q) & 1) zl |= 1u << lbit[q];
+            }
+            uint64_t xg = 0;
+            for (int j = 0; j < RB; j++) {
+                regm |= 1u << st[j];
+                if ((xl >> st[j]) & 1) xg |= 1ull << sw.desc.spos[st[j]];
+            }
+            for (int p = 0; p < (1 << RB); p++) sw.desc.gs[p] ^= xg;
+            uint32_t fzr = 0;
+            for (int p = 0; p < (1 << RB); p++) {
+                uint32_t par = 0;
+                for (int j = 0; j < RB; j++) par ^= ((p >> j) & 1) & ((zl >> st[j]) & 1);
+                fzr |= par << p;
+            }
+            const int w = fr.w & 3;
+            if (w & 2) fzr ^= (1u << (1 << RB)) - 1;
+            sw.desc.fxl = xl & ~regm;
+            sw.desc.fzl = zl & ~regm;
+            sw.desc.fzr = fzr;
+            sw.desc.fzg = fr.z & ~smask;
+            if (sw.desc.fzl || sw.desc.fzr || sw.desc.fzg) sw.desc.fflags |= TF_SIGN;
+            if (w & 1) sw.desc.fflags |= TF_SIGN | TF_ODD;
         }
         sw.op_count = prog.size() - sw.op_begin;
         sw.desc.nops = (int32_t)sw.op_count;
@@ -1112,6 +1277,8 @@ int plan_info(int n, int prec, const spz_op *ops, uint64_t n_ops, const spz_exec
         pl.swaps = env_int("SPZ_TILE_SWAPS", 1) != 0;
         pl.plan(run);
         if (pl.bad) return fail(SPZ_ERR_ARG, "tile planner produced a gate outside the register layout");
+        out->tile_gates += pl.frame_gates;
+        out->frame_gates += pl.frame_gates;
         for (const Sweep &sw : pl.sweeps) {
             if (!sw.tile) {
                 out->plain_gates++;

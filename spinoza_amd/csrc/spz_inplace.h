@@ -134,6 +134,36 @@ __device__ __forceinline__ void ip_u(const double *k, double &a, double &b, doub
         : "s"(k[0]), "s"(k[2]), "s"(k[3]), "s"(k[4]), "s"(k[5]), "s"(k[6]), "s"(k[7]));
 }
 
+// General 2x2 complex matrix (fast sweeps only: a gate conjugated by a Pauli
+// frame).  k = u00, u01, u10, u11 as (re, im) pairs:
+//   psi0' = u00 psi0 + u01 psi1,   psi1' = u10 psi0 + u11 psi1.
+// The terms that read the old psi1 are formed first (four temporaries), then
+// psi1 and psi0 are overwritten: 16 instructions, no register copies.
+__device__ __forceinline__ void ip_mat(const double *k, double &a, double &b, double &c, double &d)
+{
+    double p, t, x, y;
+    asm volatile(
+        "v_mul_f64 %0, %12, %5\n\t"      // u10r b
+        "v_fma_f64 %0, %13, %4, %0\n\t"  //  + u10i a
+        "v_fma_f64 %0, %15, %6, %0\n\t"  //  + u11i c           (d' less u11r d)
+        "v_mul_f64 %1, %12, %4\n\t"      // u10r a
+        "v_fma_f64 %1, -%13, %5, %1\n\t" //  - u10i b
+        "v_fma_f64 %1, -%15, %7, %1\n\t" //  - u11i d           (c' less u11r c)
+        "v_mul_f64 %2, %10, %6\n\t"      // u01r c
+        "v_fma_f64 %2, -%11, %7, %2\n\t" //  - u01i d
+        "v_mul_f64 %3, %10, %7\n\t"      // u01r d
+        "v_fma_f64 %3, %11, %6, %3\n\t"  //  + u01i c
+        "v_fma_f64 %6, %14, %6, %1\n\t"  // c' = u11r c + .
+        "v_fma_f64 %7, %14, %7, %0\n\t"  // d' = u11r d + .
+        "v_fma_f64 %3, %9, %4, %3\n\t"   //  + u00i a
+        "v_fma_f64 %2, -%9, %5, %2\n\t"  //  - u00i b
+        "v_fma_f64 %4, %8, %4, %2\n\t"   // a' = u00r a + .
+        "v_fma_f64 %5, %8, %5, %3"       // b' = u00r b + .
+        : "=&v"(p), "=&v"(t), "=&v"(x), "=&v"(y), "+v"(a), "+v"(b), "+v"(c), "+v"(d)
+        : "s"(k[0]), "s"(k[1]), "s"(k[2]), "s"(k[3]), "s"(k[4]), "s"(k[5]), "s"(k[6]),
+          "s"(k[7]));
+}
+
 // RZ multiply in place (gates.rs:940-946): x = fma(x, mr, -y mi),
 // y = fma(x, mi, y mr); y is formed first, while x still holds its input.
 template <bool VCOEF>

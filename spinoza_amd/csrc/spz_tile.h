@@ -47,6 +47,9 @@ enum TileGroup : uint32_t {
     // fast sweeps (AUTO): an H whose 1/sqrt(2) is applied in place, scaled by
     // c[0]; plain TG_H then adds and subtracts only (the factor is carried)
     TG_HS = 1u << 20,
+    // fast sweeps: a general 2x2 complex matrix (c[0..7], row-major, re/im
+    // interleaved), for gates conjugated by the sweep's Pauli frame
+    TG_MAT = 1u << 21,
 };
 constexpr uint32_t TG_ANYLAYOUT = TG_LAYOUT | TG_SW16 | TG_SW32;
 
@@ -100,6 +103,17 @@ static_assert(sizeof(TileOp) == 128, "TileOp must stay 128 bytes");
 constexpr int kTileMaxK = 14;
 constexpr int kTileMaxR = 32;
 
+// Pauli frame of a fast sweep.  The program leaves phi in registers, and the
+// state to store is psi = w X^x Z^z phi (w = i^k; X^x flips the qubits of x,
+// Z^z negates amplitudes whose index has odd parity on z):
+//   psi[j ^ x] = w (-1)^popcount(j & z) phi[j].
+// The host splits x and z by where the store layout holds each bit: x on
+// register bits is folded into gs[], x on lane bits is fxl; z on lane bits is
+// fzl, on register bits the per-register parity mask fzr (bit p), outside the
+// tile fzg (global bits).  The real sign of w is folded into fzr; an
+// imaginary w (k odd) stores (-im, re).
+enum TileFrameFlags : uint32_t { TF_SIGN = 1u, TF_ODD = 2u };
+
 struct TileDesc {
     int8_t spos[16];          // qubit of local bit i (ascending), K entries
     int8_t tin[16];           // local bit of thread-index bit i in the load layout
@@ -107,7 +121,10 @@ struct TileDesc {
     uint64_t gs[kTileMaxR];   // ... and in the store layout (its lane part follows the
                               // program's last layout change)
     int32_t nops;
-    int32_t pad;
+    uint32_t fflags;          // TileFrameFlags (0: no sign or phase at the store)
+    uint32_t fxl, fzl, fzr;   // Pauli frame (fast sweeps; zero otherwise)
+    uint32_t pad;
+    uint64_t fzg;
 };
 
 // fast: the AUTO arithmetic (FMA-form RX/RY, H with the factor carried; see

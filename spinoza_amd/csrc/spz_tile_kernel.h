@@ -25,6 +25,8 @@
 // bit-identical to one launch per gate.  FAST (AUTO sweeps) trades that for
 // fewer instructions in H, RX and RY.  Layout changes move data only (LDS
 // transposes, or in-wavefront lane swaps) and never change a value.
+#include <type_traits>
+
 #include "spz_inplace.h"
 #include "spz_internal.h"
 #include "spz_tile.h"
@@ -66,7 +68,8 @@ __device__ __forceinline__ void ip_pair(const double *k, double &a, double &b, d
     } else if constexpr (AK == AK_RY) {
         if constexpr (FAST) ip_ry_fma(k[0], k[1], a, b, c, d);
         else ip_ry(k[0], k[1], a, b, c, d);
-    } else ip_u(k, a, b, c, d);
+    } else if constexpr (AK == AK_MAT) ip_mat(k, a, b, c, d);
+    else ip_u(k, a, b, c, d);
 }
 
 // Lane-bit exchange of one register pair: the halves of x (register bit 0)
@@ -156,10 +159,12 @@ __device__ __forceinline__ void pair_kinds(uint32_t g, uint32_t sel, double *vr,
         if constexpr (FAST)
             if (hit(g & TG_HS)) pair_sel<R, RB, AK_HSCALED, PRED, FAST>(sel, vr, vi, k, pm);
     }
-    if (hit(g & (TG_RX | TG_RY | TG_U))) {
+    if (hit(g & (TG_RX | TG_RY | TG_U | TG_MAT))) {
         if (hit(g & TG_RX)) pair_sel<R, RB, AK_RX, PRED, FAST>(sel, vr, vi, k, pm);
         if (hit(g & TG_RY)) pair_sel<R, RB, AK_RY, PRED, FAST>(sel, vr, vi, k, pm);
         if (hit(g & TG_U)) pair_sel<R, RB, AK_U, PRED, FAST>(sel, vr, vi, k, pm);
+        if constexpr (FAST)
+            if (hit(g & TG_MAT)) pair_sel<R, RB, AK_MAT, PRED, FAST>(sel, vr, vi, k, pm);
     }
 }
 
@@ -266,6 +271,12 @@ __device__ __forceinline__ void diag_lane_kinds(uint32_t g, double *vr, double *
     if (hit(g & TG_NEG)) diag_lane<R, AK_NEG, PRED>(vr, vi, k, bit, pm);
     if (hit(g & TG_PHASE)) diag_lane<R, AK_PHASE, PRED>(vr, vi, k, bit, pm);
     if (hit(g & TG_RZ)) diag_lane<R, AK_RZ, PRED>(vr, vi, k, bit, pm);
+}
+
+// x with its sign bit flipped where bit 31 of m is set (exact, like -x).
+__device__ __forceinline__ double flip_sign(double x, uint32_t m)
+{
+    return __builtin_bit_cast(double, __builtin_bit_cast(uint64_t, x) ^ ((uint64_t)m << 32));
 }
 
 // Controls on lane bits, evaluated inside each predicated path (the opaque
@@ -401,16 +412,47 @@ k_tile(T *__restrict__ re, T *__restrict__ im, const TileDesc d, const TileOp *_
         }
     }
 
-    // store layout: the lane part is where the last layout change left it
+    // store layout: the lane part is where the last layout change left it;
+    // fast sweeps apply their Pauli frame here (see TileDesc): the X part
+    // moves each amplitude to its flipped index (lane bits here, register
+    // bits folded into gs[]), the Z part and the phase set its sign, and an
+    // imaginary phase stores (-im, re)
+    const uint32_t lphi = lt;
+    if constexpr (FAST) lt ^= d.fxl;
     uint64_t gs = 0;
 #pragma unroll
     for (int b = 0; b < K; b++)
         if ((lt >> b) & 1u) gs |= 1ull << d.spos[b];
+    auto store = [&](auto sign, auto odd) {
+        uint32_t sl = 0;  // bit 31: odd parity of the lane and tile parts on z
+        if constexpr (decltype(sign)::value)
+            sl = (uint32_t)(__builtin_popcount(lphi & d.fzl) +
+                            __builtin_popcountll(base & d.fzg)) << 31;
 #pragma unroll
-    for (int p = 0; p < R; p++) {
-        const uint64_t g = base | gs | d.gs[p];
-        re[g] = (T)vr[p];
-        im[g] = (T)vi[p];
+        for (int p = 0; p < R; p++) {
+            const uint64_t g = base | gs | d.gs[p];
+            double a = vr[p], b = vi[p];
+            if constexpr (decltype(odd)::value) {
+                a = vi[p];
+                b = vr[p];
+            }
+            if constexpr (decltype(sign)::value) {
+                const uint32_t m = sl ^ (((d.fzr >> p) & 1u) << 31);
+                a = flip_sign(a, decltype(odd)::value ? m ^ 0x80000000u : m);
+                b = flip_sign(b, m);
+            }
+            re[g] = (T)a;
+            im[g] = (T)b;
+        }
+    };
+    using F = std::false_type;
+    using Tr = std::true_type;
+    if constexpr (FAST) {
+        if (d.fflags & TF_ODD) store(Tr{}, Tr{});
+        else if (d.fflags & TF_SIGN) store(Tr{}, F{});
+        else store(F{}, F{});
+    } else {
+        store(F{}, F{});
     }
 }
 

@@ -57,7 +57,8 @@ def run_ops(state, ops, mode=N.EXEC_AUTO, tracker=None, tile_qubits=0, encoded=N
 def plan_stats(n, ops, mode=N.EXEC_AUTO, tile_qubits=0, precision="f64"):
     """How the tile executor splits `ops` for an n-qubit state (host only, no
     device work): dict with sweeps, tile_gates, layouts (LDS transposes),
-    lane_swaps (in-wavefront layout changes) and plain_gates."""
+    lane_swaps (in-wavefront layout changes), frame_gates (X, Y, Z absorbed
+    into a sweep's Pauli frame, counted in tile_gates too) and plain_gates."""
     arr, keep = encode_ops(ops)
     opts = N.ExecOpts_t()
     opts.mode = mode
@@ -68,4 +69,5 @@ def plan_stats(n, ops, mode=N.EXEC_AUTO, tile_qubits=0, precision="f64"):
                                    ctypes.byref(info)))
     del keep
     return {"sweeps": info.sweeps, "tile_gates": info.tile_gates, "layouts": info.layouts,
-            "lane_swaps": info.lane_swaps, "plain_gates": info.plain_gates}
+            "lane_swaps": info.lane_swaps, "frame_gates": info.frame_gates,
+            "plain_gates": info.plain_gates}

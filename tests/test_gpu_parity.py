@@ -484,3 +484,55 @@ def test_lane_swaps_and_fast_sweeps_vs_oracle(spz, orc, tile_qubits):
         s = to_gpu(spz, base)
         run_ops(s, ops, mode=mode, tile_qubits=tile_qubits)
         assert_match(s, ref, exact=(mode == N.EXEC_STRICT), what=f"swaps/fast mode={mode}")
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("tile_qubits,precision", [(10, "f64"), (12, "f64"), (11, "f32")])
+def test_pauli_frames_vs_oracle(spz, orc, tile_qubits, precision):
+    """AUTO sweeps absorb uncontrolled X, Y and Z into a Pauli frame
+    (psi = i^w X^x Z^z phi), carry it through H, CX and CZ, conjugate every
+    other gate by it (RX/RY/RZ/U sign changes, P and controlled Z under X as
+    general diagonals, U under X and controlled X/Y/H as general matrices,
+    controls under X inverted), and apply it at the store (index XOR, sign
+    parity, an imaginary phase as (-im, re)).  Circuits rich in Paulis, with
+    CX/CZ whose control lies outside the tile (Z frame bits outside the
+    tile), zero-controls, multi-controls and a Pauli-only tail.  FP64 within
+    1e-12 of the oracle; FP32 storage within 1e-6."""
+    print(f"case pauli frames K={tile_qubits} {precision}", flush=True)
+    from spinoza_amd import _native as N
+    from spinoza_amd.executor import plan_stats, run_ops
+    from spinoza_amd.workloads import op
+    n = 14
+    rng = np.random.default_rng(7 + tile_qubits)
+    single = [N.X, N.Y, N.Z, N.X, N.Y, N.Z, N.H, N.RX, N.RY, N.RZ, N.P, N.U]
+    ops = []
+
+    def ang():
+        return float(rng.uniform(0, 6.28))
+
+    for layer in range(10):
+        for q in range(n):
+            ops.append(op(single[int(rng.integers(len(single)))], q, theta=ang(), phi=ang(),
+                          lam=ang()))
+        perm = rng.permutation(n)
+        for p in range(n // 2):
+            c, t = int(perm[2 * p]), int(perm[2 * p + 1])
+            ops.append(op(N.X if rng.integers(2) else N.Z, t, ones=1 << c))
+        for _ in range(3):
+            c1, c2, t = (int(x) for x in rng.choice(n, 3, replace=False))
+            k = [N.U, N.Y, N.H, N.P, N.RX, N.X, N.Z][int(rng.integers(7))]
+            ones, zeros = [(1 << c1, 1 << c2), (1 << c1 | 1 << c2, 0), (0, 1 << c1),
+                           (1 << c1, 0)][int(rng.integers(4))]
+            ops.append(op(k, t, theta=ang(), phi=ang(), lam=ang(), ones=ones, zeros=zeros))
+    for _ in range(25):  # Pauli-only tail, an odd number of Y
+        ops.append(op([N.X, N.Z][int(rng.integers(2))], int(rng.integers(n))))
+    ops.append(op(N.Y, int(rng.integers(n))))
+    st = plan_stats(n, ops, mode=N.EXEC_AUTO, tile_qubits=tile_qubits, precision=precision)
+    assert st["frame_gates"] > 0 and st["tile_gates"] + st["plain_gates"] == len(ops), st
+    base = orc.random_state(3 + tile_qubits, n)
+    ref = base.copy()
+    orc.execute(ref, ops)
+    s = to_gpu(spz, base, precision)
+    run_ops(s, ops, mode=N.EXEC_AUTO, tile_qubits=tile_qubits)
+    assert_match(s, ref, exact=False, tol=TOL if precision == "f64" else 1e-6,
+                 what=f"pauli frames K={tile_qubits} {precision}")
