@@ -17,6 +17,7 @@
 
 #include <algorithm>
 #include <mutex>
+#include <cmath>
 #include <vector>
 
 #include "spz_internal.h"
@@ -238,6 +239,74 @@ void conj_coef(Coef &k, bool a, bool b)
     memcpy(k.c, m, sizeof m);
 }
 
+// The fast sweeps' rotation and phase forms (shears): a rotation by a, |a| <=
+// pi/2, given by (cos a, sin a) with cos a >= 0, becomes (-tan(a/2), sin a).
+// RX(theta) and RY(theta) rotate by a = theta/2; a phase e^(ib) (P, and each
+// half of a controlled general diagonal) by b.  A larger angle a is a - pi
+// and a sign: an uncontrolled RX/RY then runs as -G (sign 1: the caller adds
+// a global -1), an uncontrolled P(b) as Z P(b - pi) (sign 2: the caller puts
+// a Z on the target after it); a controlled one runs twice at half the angle
+// (returns 2).  Other kinds are left as they are (returns 1).
+void shear_of(double c, double s, double *k)
+{
+    k[0] = -s / (1.0 + c);
+    k[1] = s;
+}
+
+// e^(ia/2), cos >= 0, for e^(ia) = (c, s), a in (-pi, pi]: the larger of
+// cos(a/2) and |sin(a/2)| from (1 + c) / 2 or (1 - c) / 2, the other from
+// sin a = 2 sin(a/2) cos(a/2), so that neither cancels.
+void half_of(double c, double s, double *hc, double *hs)
+{
+    if (c >= 0.0) {
+        const double h = std::sqrt(0.5 * (1.0 + c));
+        *hc = h;
+        *hs = s / (2.0 * h);
+    } else {
+        const double q = std::sqrt(0.5 * (1.0 - c));
+        *hc = std::fabs(s) / (2.0 * q);
+        *hs = std::signbit(s) ? -q : q;
+    }
+}
+
+int shear_form(Coef &k, bool unc, int *sign)
+{
+    *sign = 0;
+    double c, s;
+    switch (k.ak) {
+    case AK_RX: c = k.c[0]; s = -k.c[1]; break;
+    case AK_RY: c = k.c[1]; s = k.c[0]; break;
+    case AK_PHASE: c = k.c[0]; s = k.c[1]; break;
+    case AK_RZ: {  // controlled general diagonal: both halves
+        double c0 = k.c[0], s0 = k.c[1], c1 = k.c[2], s1 = k.c[3];
+        const int reps = (c0 < 0.0 || c1 < 0.0) ? 2 : 1;
+        if (reps == 2) {
+            half_of(c0, s0, &c0, &s0);
+            half_of(c1, s1, &c1, &s1);
+        }
+        memset(k.c, 0, sizeof k.c);
+        shear_of(c0, s0, k.c);
+        shear_of(c1, s1, k.c + 2);
+        return reps;
+    }
+    default: return 1;
+    }
+    int reps = 1;
+    if (c < 0.0) {
+        if (unc) {
+            c = -c;
+            s = -s;
+            *sign = k.ak == AK_PHASE ? 2 : 1;
+        } else {
+            half_of(c, s, &c, &s);
+            reps = 2;
+        }
+    }
+    memset(k.c, 0, sizeof k.c);
+    shear_of(c, s, k.c);
+    return reps;
+}
+
 struct Planner {
     int n, K, LOGW, RB, klo;
     int improve = 0;  // local-search passes over the tile set (AUTO only)
@@ -246,6 +315,7 @@ struct Planner {
     bool swaps = true;   // in-wavefront layout changes when they reach the set
     bool bad = false;
     uint64_t frame_gates = 0;  // gates absorbed into Pauli frames
+    uint64_t split_ops = 0;    // second halves of gates run as two half-angle ops
     std::vector<TileOp> prog;
     std::vector<Sweep> sweeps;
 
@@ -719,14 +789,11 @@ struct Planner {
         };
         reg_offsets(RS, sw.desc.gr);
         for (int t = 0; t < LOGW; t++) sw.desc.tin[t] = (int8_t)THR[t];
-        // fast sweeps: uncontrolled H ops leave their 1/sqrt(2) to the next
-        // uncontrolled RX/RY/U/RZ (whose coefficients take the product) or,
-        // at the end of the sweep, to the last of them (a scaled H)
-        std::vector<size_t> pend_h;
-        auto factor = [](size_t d) {
-            return (d & 1) ? std::ldexp(0.70710678118654752440, -(int)(d / 2))
-                           : std::ldexp(1.0, -(int)(d / 2));
-        };
+        // fast sweeps: uncontrolled H ops leave out their 1/sqrt(2) and
+        // uncontrolled diagonals their phase on |0>; both go into one global
+        // factor (gre, gim) that the store applies
+        size_t hdebt = 0;
+        double gre = 1.0, gim = 0.0;
         Frame fr;
         size_t li_next = 0;
         for (int gi : seq) {
@@ -783,18 +850,34 @@ struct Planner {
                     conj_coef(g.k, fa, fb);
                 }
             }
+            // fast arithmetic: rotations and phases as shears (see
+            // shear_form); a controlled one beyond pi/2 runs as two halves
+            bool zafter = false;
+            int reps = 1;
+            if (fast) {
+                const bool unc = !g.ones && !g.zeros;
+                if (g.k.ak == AK_RZ && unc) {  // d0 to the global factor, d1/d0 on |1>
+                    const double d0r = g.k.c[0], d0i = g.k.c[1], d1r = g.k.c[2], d1i = g.k.c[3];
+                    const double t = gre * d0r - gim * d0i;
+                    gim = gre * d0i + gim * d0r;
+                    gre = t;
+                    memset(g.k.c, 0, sizeof g.k.c);
+                    g.k.ak = AK_PHASE;
+                    g.k.c[0] = d1r * d0r + d1i * d0i;
+                    g.k.c[1] = d1i * d0r - d1r * d0i;
+                }
+                int sign = 0;
+                reps = shear_form(g.k, unc, &sign);
+                if (sign == 1) fr.w += 2;     // the op ran as -G: a global -1
+                if (sign == 2) zafter = true; // P ran as Z P: Z on phi (after it)
+                split_ops += reps - 1;
+            }
+            for (int rep = 0; rep < reps; rep++) {
             TileOp op;
             memset(&op, 0, sizeof op);
             op.ak = g.k.ak;
             for (int j = 0; j < 8; j++) op.c[j] = g.k.c[j];
             const bool uncontrolled = !g.ones && !g.zeros;
-            if (fast && uncontrolled && !pend_h.empty() &&
-                (g.k.ak == AK_RX || g.k.ak == AK_RY || g.k.ak == AK_U || g.k.ak == AK_RZ ||
-                 g.k.ak == AK_MAT)) {
-                const double f = factor(pend_h.size());
-                for (int j = 0; j < 8; j++) op.c[j] *= f;
-                pend_h.clear();
-            }
             auto slot_of = [&](int b) {
                 for (int j = 0; j < RB; j++)
                     if (RS[j] == b) return j;
@@ -827,7 +910,7 @@ struct Planner {
             }
             op.grp = TG_GATE | kbit;
             if (fast && g.k.ak == AK_H) {
-                if (uncontrolled) pend_h.push_back(prog.size());
+                if (uncontrolled) hdebt++;
                 else op.grp = (op.grp & ~TG_H) | TG_HS;  // applies its own factor
             }
             if (g.diag) {
@@ -858,6 +941,8 @@ struct Planner {
                 op.grp |= pred ? TG_PAIRP : TG_PAIR;
             }
             prog.push_back(op);
+            }
+            if (zafter) fr.z ^= tb;  // (on phi: the frame takes it on its right)
             if (push == PUSH_H) {  // H X^a Z^b = (-1)^(ab) X^b Z^a H
                 fr.x = (fr.x & ~tb) | (fb ? tb : 0);
                 fr.z = (fr.z & ~tb) | (fa ? tb : 0);
@@ -874,10 +959,16 @@ struct Planner {
                 if (fa) fr.z ^= cb;
             }
         }
-        if (!pend_h.empty()) {
-            TileOp &h = prog[pend_h.back()];
-            h.grp = (h.grp & ~TG_H) | TG_HS;
-            h.c[0] = factor(pend_h.size());
+        if (fast) {
+            const double f = (hdebt & 1) ? std::ldexp(0.70710678118654752440, -(int)(hdebt / 2))
+                                         : std::ldexp(1.0, -(int)(hdebt / 2));
+            gre *= f;
+            gim *= f;
+            if (gre != 1.0 || gim != 0.0) {
+                sw.desc.gre = gre;
+                sw.desc.gim = gim;
+                sw.desc.fflags |= TF_PHASE;
+            }
         }
         // store layout: the last one, unless it holds a low bit in registers
         // (then those register bits are swapped for unused high bits) or the
@@ -1278,6 +1369,7 @@ int plan_info(int n, int prec, const spz_op *ops, uint64_t n_ops, const spz_exec
         pl.plan(run);
         if (pl.bad) return fail(SPZ_ERR_ARG, "tile planner produced a gate outside the register layout");
         out->tile_gates += pl.frame_gates;
+        out->tile_gates -= pl.split_ops;
         out->frame_gates += pl.frame_gates;
         for (const Sweep &sw : pl.sweeps) {
             if (!sw.tile) {

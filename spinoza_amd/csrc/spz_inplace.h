@@ -262,4 +262,77 @@ __device__ __forceinline__ void ip_ry_fma(double sn, double cs, double &a, doubl
         : "s"(sn), "s"(cs));
 }
 
+// Rotations as three shears (fast sweeps): the real pair (x, y) turned by an
+// angle a with |a| <= pi/2,
+//   x += k1 y;  y += k2 x;  x += k1 y      (k1 = -tan(a/2), k2 = sin a),
+// three FMAs in place and no temporaries (the host keeps |a| <= pi/2, where
+// |k1| <= 1, and carries the sign of a larger angle elsewhere).  NEG turns by
+// -a (the coefficients negated through the operand modifiers).  Two pairs are
+// interleaved so that neither waits on its own FMA chain.
+template <bool NEG0, bool NEG1>
+__device__ __forceinline__ void ip_rot2(double k1, double k2, double &x0, double &y0, double &x1,
+                                        double &y1)
+{
+#define SPZ_ROT_ASM(N0, N1)                                                    \
+    asm volatile("v_fma_f64 %0, " N0 "%4, %1, %0\n\t"                        \
+                 "v_fma_f64 %2, " N1 "%4, %3, %2\n\t"                        \
+                 "v_fma_f64 %1, " N0 "%5, %0, %1\n\t"                        \
+                 "v_fma_f64 %3, " N1 "%5, %2, %3\n\t"                        \
+                 "v_fma_f64 %0, " N0 "%4, %1, %0\n\t"                        \
+                 "v_fma_f64 %2, " N1 "%4, %3, %2"                              \
+                 : "+v"(x0), "+v"(y0), "+v"(x1), "+v"(y1)                      \
+                 : "s"(k1), "s"(k2))
+    if constexpr (!NEG0 && !NEG1) SPZ_ROT_ASM("", "");
+    else if constexpr (NEG0 && !NEG1) SPZ_ROT_ASM("-", "");
+    else if constexpr (!NEG0 && NEG1) SPZ_ROT_ASM("", "-");
+    else SPZ_ROT_ASM("-", "-");
+#undef SPZ_ROT_ASM
+}
+
+// One pair, coefficients per lane (VCOEF, in VGPRs) or uniform.
+template <bool VCOEF>
+__device__ __forceinline__ void ip_rot(double k1, double k2, double &x, double &y)
+{
+    if constexpr (VCOEF)
+        asm volatile("v_fma_f64 %0, %2, %1, %0\n\t"
+                     "v_fma_f64 %1, %3, %0, %1\n\t"
+                     "v_fma_f64 %0, %2, %1, %0"
+                     : "+v"(x), "+v"(y)
+                     : "v"(k1), "v"(k2));
+    else
+        asm volatile("v_fma_f64 %0, %2, %1, %0\n\t"
+                     "v_fma_f64 %1, %3, %0, %1\n\t"
+                     "v_fma_f64 %0, %2, %1, %0"
+                     : "+v"(x), "+v"(y)
+                     : "s"(k1), "s"(k2));
+}
+
+// RX(theta) with a = theta/2: (re psi0, im psi1) turns by -a and
+// (im psi0, re psi1) by +a; RY(theta): (re psi0, re psi1) and
+// (im psi0, im psi1) both by +a.  k = (-tan(a/2), sin a).
+__device__ __forceinline__ void ip_rx_shear(double k1, double k2, double &a, double &b, double &c,
+                                            double &d)
+{
+    ip_rot2<true, false>(k1, k2, a, d, b, c);
+}
+
+__device__ __forceinline__ void ip_ry_shear(double k1, double k2, double &a, double &b, double &c,
+                                            double &d)
+{
+    ip_rot2<false, false>(k1, k2, a, c, b, d);
+}
+
+// (x, y) <- (x, y) (gr + i gi): the fast sweeps' global factor at the store.
+__device__ __forceinline__ void ip_cmul(double gr, double gi, double &x, double &y)
+{
+    double t;
+    asm volatile(
+        "v_mul_f64 %0, %4, %2\n\t"      // gi y
+        "v_mul_f64 %2, %3, %2\n\t"      // gr y
+        "v_fma_f64 %2, %4, %1, %2\n\t"  // y' = gi x + gr y
+        "v_fma_f64 %1, %3, %1, -%0"       // x' = gr x - gi y
+        : "=&v"(t), "+v"(x), "+v"(y)
+        : "s"(gr), "s"(gi));
+}
+
 }  // namespace spz

@@ -112,7 +112,7 @@ constexpr int kTileMaxR = 32;
 // fzl, on register bits the per-register parity mask fzr (bit p), outside the
 // tile fzg (global bits).  The real sign of w is folded into fzr; an
 // imaginary w (k odd) stores (-im, re).
-enum TileFrameFlags : uint32_t { TF_SIGN = 1u, TF_ODD = 2u };
+enum TileFrameFlags : uint32_t { TF_SIGN = 1u, TF_ODD = 2u, TF_PHASE = 4u };
 
 struct TileDesc {
     int8_t spos[16];          // qubit of local bit i (ascending), K entries
@@ -125,6 +125,9 @@ struct TileDesc {
     uint32_t fxl, fzl, fzr;   // Pauli frame (fast sweeps; zero otherwise)
     uint32_t pad;
     uint64_t fzg;
+    double gre, gim;          // fast sweeps, TF_PHASE: a global factor applied before
+                              // the frame (the phases the sweep's diagonals left out
+                              // and the 1/sqrt(2) of its plain H)
 };
 
 // fast: the AUTO arithmetic (FMA-form RX/RY, H with the factor carried; see

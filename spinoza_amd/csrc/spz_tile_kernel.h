@@ -49,9 +49,9 @@ __device__ __forceinline__ bool hit(uint32_t bits) { return __builtin_expect(bit
 // H ops leave the factor to a later gate.
 constexpr int AK_HSCALED = 100;
 
-// FAST selects the arithmetic of AUTO sweeps: RX and RY with one multiply and
-// one FMA per output (the reference rounds both products), H as a sum and a
-// difference whose 1/sqrt(2) the host folds into a later gate's coefficients.
+// FAST selects the arithmetic of AUTO sweeps: RX and RY as two rotations of
+// real pairs, three FMAs each (shears; the host gives (-tan(a/2), sin a)), H
+// as a sum and a difference whose 1/sqrt(2) the host carries to the store.
 // Without FAST every body is the reference's operation sequence.
 template <int AK, bool FAST>
 __device__ __forceinline__ void ip_pair(const double *k, double &a, double &b, double &c, double &d)
@@ -63,10 +63,10 @@ __device__ __forceinline__ void ip_pair(const double *k, double &a, double &b, d
     else if constexpr (AK == AK_X) { ip_swap(a, c); ip_swap(b, d); }
     else if constexpr (AK == AK_Y) ip_y(a, b, c, d);
     else if constexpr (AK == AK_RX) {
-        if constexpr (FAST) ip_rx_fma(k[0], k[1], a, b, c, d);
+        if constexpr (FAST) ip_rx_shear(k[0], k[1], a, b, c, d);
         else ip_rx(k[0], k[1], a, b, c, d);
     } else if constexpr (AK == AK_RY) {
-        if constexpr (FAST) ip_ry_fma(k[0], k[1], a, b, c, d);
+        if constexpr (FAST) ip_ry_shear(k[0], k[1], a, b, c, d);
         else ip_ry(k[0], k[1], a, b, c, d);
     } else if constexpr (AK == AK_MAT) ip_mat(k, a, b, c, d);
     else ip_u(k, a, b, c, d);
@@ -171,21 +171,32 @@ __device__ __forceinline__ void pair_kinds(uint32_t g, uint32_t sel, double *vr,
 // Diagonal update of one amplitude whose target bit is known at compile
 // time (BITK) or given per lane / per tile (bit, with per-lane multipliers
 // mr, mi for RZ).
-template <int AK>
+// FAST: the phases are rotations of (re, im) as three shears, the host
+// giving (-tan(b/2), sin b) for a phase e^(ib), |b| <= pi/2.
+template <int AK, bool FAST>
 __device__ __forceinline__ void diag_known(const double *k, bool bit, double &x, double &y)
 {
     if constexpr (AK == AK_NEG) {
         if (bit) { x = -x; y = -y; }
     } else if constexpr (AK == AK_PHASE) {
-        if (bit) ip_phase(k[0], k[1], x, y);
+        if constexpr (FAST) {
+            if (bit) ip_rot<false>(k[0], k[1], x, y);
+        } else {
+            if (bit) ip_phase(k[0], k[1], x, y);
+        }
     } else {  // AK_RZ: diag[bit]
-        if (bit) ip_rz<false>(k[2], k[3], x, y);
-        else ip_rz<false>(k[0], k[1], x, y);
+        if constexpr (FAST) {
+            if (bit) ip_rot<false>(k[2], k[3], x, y);
+            else ip_rot<false>(k[0], k[1], x, y);
+        } else {
+            if (bit) ip_rz<false>(k[2], k[3], x, y);
+            else ip_rz<false>(k[0], k[1], x, y);
+        }
     }
 }
 
 // Diagonal target on register bit J.
-template <int R, int AK, int J, bool PRED>
+template <int R, int AK, int J, bool PRED, bool FAST>
 __device__ __forceinline__ void diag_j(double *vr, double *vi, const double *k, uint32_t pm)
 {
 #pragma unroll
@@ -194,31 +205,45 @@ __device__ __forceinline__ void diag_j(double *vr, double *vi, const double *k, 
         if constexpr (AK == AK_NEG || AK == AK_PHASE)
             if (!bit) continue;  // |0> amplitudes are untouched
         if constexpr (!PRED) {
-            diag_known<AK>(k, bit, vr[p], vi[p]);
+            diag_known<AK, FAST>(k, bit, vr[p], vi[p]);
         } else {
             if ((pm >> p) & 1u)
-                diag_known<AK>(k, bit, vr[p], vi[p]);
+                diag_known<AK, FAST>(k, bit, vr[p], vi[p]);
         }
     }
 }
 
-template <int R, int RB, int AK, bool PRED>
+template <int R, int RB, int AK, bool PRED, bool FAST>
 __device__ __forceinline__ void diag_sel(uint32_t sel, double *vr, double *vi, const double *k,
                                          uint32_t pm)
 {
-    if (hit(sel & 1u)) diag_j<R, AK, 0, PRED>(vr, vi, k, pm);
-    if constexpr (RB > 1) if (hit(sel & 2u)) diag_j<R, AK, 1, PRED>(vr, vi, k, pm);
-    if constexpr (RB > 2) if (hit(sel & 4u)) diag_j<R, AK, 2, PRED>(vr, vi, k, pm);
-    if constexpr (RB > 3) if (hit(sel & 8u)) diag_j<R, AK, 3, PRED>(vr, vi, k, pm);
+    if (hit(sel & 1u)) diag_j<R, AK, 0, PRED, FAST>(vr, vi, k, pm);
+    if constexpr (RB > 1) if (hit(sel & 2u)) diag_j<R, AK, 1, PRED, FAST>(vr, vi, k, pm);
+    if constexpr (RB > 2) if (hit(sel & 4u)) diag_j<R, AK, 2, PRED, FAST>(vr, vi, k, pm);
+    if constexpr (RB > 3) if (hit(sel & 8u)) diag_j<R, AK, 3, PRED, FAST>(vr, vi, k, pm);
 }
 
 // Diagonal target on a lane bit or outside the tile: `bit` per lane (or per
 // tile); the RZ multipliers are selected once per op.
-template <int R, int AK, bool PRED>
+template <int R, int AK, bool PRED, bool FAST>
 __device__ __forceinline__ void diag_lane(double *vr, double *vi, const double *k, bool bit,
                                           uint32_t pm)
 {
-    if constexpr (AK == AK_RZ) {
+    if constexpr (FAST && AK != AK_NEG) {
+        // one rotation per amplitude with per-lane shear coefficients (a
+        // phase on |1> turns the |0> lanes by zero: exact)
+        const double k1 = AK == AK_RZ ? (bit ? k[2] : k[0]) : (bit ? k[0] : 0.0);
+        const double k2 = AK == AK_RZ ? (bit ? k[3] : k[1]) : (bit ? k[1] : 0.0);
+#pragma unroll
+        for (int p = 0; p < R; p++) {
+            if constexpr (!PRED) {
+                ip_rot<true>(k1, k2, vr[p], vi[p]);
+            } else {
+                if ((pm >> p) & 1u)
+                    ip_rot<true>(k1, k2, vr[p], vi[p]);
+            }
+        }
+    } else if constexpr (AK == AK_RZ) {
         const double mr = bit ? k[2] : k[0];
         const double mi = bit ? k[3] : k[1];
 #pragma unroll
@@ -246,31 +271,31 @@ __device__ __forceinline__ void diag_lane(double *vr, double *vi, const double *
 #pragma unroll
         for (int p = 0; p < R; p++) {
             if constexpr (!PRED) {
-                diag_known<AK>(k, true, vr[p], vi[p]);
+                diag_known<AK, false>(k, true, vr[p], vi[p]);
             } else {
                 if ((pm >> p) & 1u)
-                    diag_known<AK>(k, true, vr[p], vi[p]);
+                    diag_known<AK, false>(k, true, vr[p], vi[p]);
             }
         }
     }
 }
 
-template <int R, int RB, bool PRED>
+template <int R, int RB, bool PRED, bool FAST>
 __device__ __forceinline__ void diag_reg_kinds(uint32_t g, uint32_t sel, double *vr, double *vi,
                                                const double *k, uint32_t pm)
 {
-    if (hit(g & TG_NEG)) diag_sel<R, RB, AK_NEG, PRED>(sel, vr, vi, k, pm);
-    if (hit(g & TG_PHASE)) diag_sel<R, RB, AK_PHASE, PRED>(sel, vr, vi, k, pm);
-    if (hit(g & TG_RZ)) diag_sel<R, RB, AK_RZ, PRED>(sel, vr, vi, k, pm);
+    if (hit(g & TG_NEG)) diag_sel<R, RB, AK_NEG, PRED, FAST>(sel, vr, vi, k, pm);
+    if (hit(g & TG_PHASE)) diag_sel<R, RB, AK_PHASE, PRED, FAST>(sel, vr, vi, k, pm);
+    if (hit(g & TG_RZ)) diag_sel<R, RB, AK_RZ, PRED, FAST>(sel, vr, vi, k, pm);
 }
 
-template <int R, bool PRED>
+template <int R, bool PRED, bool FAST>
 __device__ __forceinline__ void diag_lane_kinds(uint32_t g, double *vr, double *vi,
                                                 const double *k, bool bit, uint32_t pm)
 {
-    if (hit(g & TG_NEG)) diag_lane<R, AK_NEG, PRED>(vr, vi, k, bit, pm);
-    if (hit(g & TG_PHASE)) diag_lane<R, AK_PHASE, PRED>(vr, vi, k, bit, pm);
-    if (hit(g & TG_RZ)) diag_lane<R, AK_RZ, PRED>(vr, vi, k, bit, pm);
+    if (hit(g & TG_NEG)) diag_lane<R, AK_NEG, PRED, FAST>(vr, vi, k, bit, pm);
+    if (hit(g & TG_PHASE)) diag_lane<R, AK_PHASE, PRED, FAST>(vr, vi, k, bit, pm);
+    if (hit(g & TG_RZ)) diag_lane<R, AK_RZ, PRED, FAST>(vr, vi, k, bit, pm);
 }
 
 // x with its sign bit flipped where bit 31 of m is set (exact, like -x).
@@ -396,29 +421,36 @@ k_tile(T *__restrict__ re, T *__restrict__ im, const TileDesc d, const TileOp *_
                         pair_kinds<R, RB, true, FAST>(g, sel, vr, vi, k, pm);
             }
             if (hit(g & (TG_DREG | TG_DREGP | TG_DLANE | TG_DLANEP))) {
-                if (hit(g & TG_DREG)) diag_reg_kinds<R, RB, false>(g, sel, vr, vi, k, 0);
+                if (hit(g & TG_DREG)) diag_reg_kinds<R, RB, false, FAST>(g, sel, vr, vi, k, 0);
                 if (hit(g & TG_DREGP))
-                    if (lane_pass(lt, cur.mlo, cur.mlz)) diag_reg_kinds<R, RB, true>(g, sel, vr, vi, k, pm);
+                    if (lane_pass(lt, cur.mlo, cur.mlz))
+                        diag_reg_kinds<R, RB, true, FAST>(g, sel, vr, vi, k, pm);
                 if (hit(g & (TG_DLANE | TG_DLANEP))) {
                     // target on a lane bit (dsel 1) or outside the tile (dsel 2)
                     const bool bit = cur.dsel == 2 ? (base & cur.ebit) != 0
                                                    : ((lt >> cur.dpos) & 1u) != 0;
-                    if (hit(g & TG_DLANE)) diag_lane_kinds<R, false>(g, vr, vi, k, bit, 0);
+                    if (hit(g & TG_DLANE)) diag_lane_kinds<R, false, FAST>(g, vr, vi, k, bit, 0);
                     if (hit(g & TG_DLANEP))
                         if (lane_pass(lt, cur.mlo, cur.mlz))
-                            diag_lane_kinds<R, true>(g, vr, vi, k, bit, pm);
+                            diag_lane_kinds<R, true, FAST>(g, vr, vi, k, bit, pm);
                 }
             }
         }
     }
 
     // store layout: the lane part is where the last layout change left it;
-    // fast sweeps apply their Pauli frame here (see TileDesc): the X part
-    // moves each amplitude to its flipped index (lane bits here, register
-    // bits folded into gs[]), the Z part and the phase set its sign, and an
-    // imaginary phase stores (-im, re)
+    // fast sweeps apply their global factor and Pauli frame here (see
+    // TileDesc): the X part moves each amplitude to its flipped index (lane
+    // bits here, register bits folded into gs[]), the Z part and the phase
+    // set its sign, and an imaginary phase stores (-im, re)
     const uint32_t lphi = lt;
-    if constexpr (FAST) lt ^= d.fxl;
+    if constexpr (FAST) {
+        lt ^= d.fxl;
+        if (d.fflags & TF_PHASE) {  // the sweep's global factor (phases, H scale)
+#pragma unroll
+            for (int p = 0; p < R; p++) ip_cmul(d.gre, d.gim, vr[p], vi[p]);
+        }
+    }
     uint64_t gs = 0;
 #pragma unroll
     for (int b = 0; b < K; b++)

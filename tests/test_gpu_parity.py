@@ -536,3 +536,42 @@ def test_pauli_frames_vs_oracle(spz, orc, tile_qubits, precision):
     run_ops(s, ops, mode=N.EXEC_AUTO, tile_qubits=tile_qubits)
     assert_match(s, ref, exact=False, tol=TOL if precision == "f64" else 1e-6,
                  what=f"pauli frames K={tile_qubits} {precision}")
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("tile_qubits", [10, 12])
+def test_fast_rotation_edge_angles_vs_oracle(spz, orc, tile_qubits):
+    """AUTO sweeps run RX/RY/RZ/P as rotations by three shears, which need
+    |angle| <= pi/2: a larger angle runs as angle - pi with its sign carried
+    (an uncontrolled RX/RY as a global -1, an uncontrolled P as a Z in the
+    frame, an uncontrolled RZ or general diagonal as a phase on |1> with the
+    |0> phase in the sweep's global factor), a controlled one as two ops at
+    half the angle.  Angles at and around the switch points (0, +-pi/2,
+    +-pi, pi +- 1e-9, 2 pi, 4 pi - 1e-9), uncontrolled, 1- and 0-controlled,
+    on every qubit of a 14-qubit state (register, lane and outside-the-tile
+    targets).  Within 1e-12 of the oracle."""
+    print(f"case fast rotation edge angles K={tile_qubits}", flush=True)
+    from spinoza_amd import _native as N
+    from spinoza_amd.executor import run_ops
+    from spinoza_amd.workloads import op
+    n = 14
+    pi = math.pi
+    angles = [0.0, pi / 2, -pi / 2, pi / 2 + 1e-12, pi, -pi, pi - 1e-9, pi + 1e-9, 1.5 * pi,
+              2 * pi, -2 * pi, 3 * pi, 4 * pi - 1e-9, 0.3, -2.9]
+    rng = np.random.default_rng(17 + tile_qubits)
+    ops = []
+    for i, th in enumerate(angles * 3):
+        for q in range(n):
+            kind = [N.RX, N.RY, N.RZ, N.P][(i + q) % 4]
+            c = (q + 1 + i % (n - 1)) % n  # never the target
+            ctl = [dict(), dict(ones=1 << c), dict(zeros=1 << c)][(i // len(angles) + q) % 3]
+            ops.append(op(kind, q, theta=th, **ctl))
+        ops.append(op(N.H, int(rng.integers(n))))
+        c, t = (int(x) for x in rng.choice(n, 2, replace=False))
+        ops.append(op(N.X, t, ones=1 << c))
+    base = orc.random_state(29 + tile_qubits, n)
+    ref = base.copy()
+    orc.execute(ref, ops)
+    s = to_gpu(spz, base)
+    run_ops(s, ops, mode=N.EXEC_AUTO, tile_qubits=tile_qubits)
+    assert_match(s, ref, exact=False, what=f"edge angles K={tile_qubits}")
