@@ -224,44 +224,6 @@ __device__ __forceinline__ void ip_h_unscaled(double &a, double &b, double &c, d
         : "+v"(a), "+v"(b), "+v"(c), "+v"(d));
 }
 
-// RX with one product and one FMA per output: a = a cs - d ns,
-// b = b cs + c ns, c = c cs - b ns, d = d cs + a ns.
-__device__ __forceinline__ void ip_rx_fma(double cs, double ns, double &a, double &b, double &c,
-                                          double &d)
-{
-    double t0, t1, t2, t3;
-    asm volatile(
-        "v_mul_f64 %0, %7, %9\n\t"         // d ns
-        "v_mul_f64 %1, %6, %9\n\t"         // c ns
-        "v_mul_f64 %2, %5, %9\n\t"         // b ns
-        "v_mul_f64 %3, %4, %9\n\t"         // a ns
-        "v_fma_f64 %4, %4, %8, -%0\n\t"
-        "v_fma_f64 %5, %5, %8, %1\n\t"
-        "v_fma_f64 %6, %6, %8, -%2\n\t"
-        "v_fma_f64 %7, %7, %8, %3"
-        : "=&v"(t0), "=&v"(t1), "=&v"(t2), "=&v"(t3), "+v"(a), "+v"(b), "+v"(c), "+v"(d)
-        : "s"(cs), "s"(ns));
-}
-
-// RY likewise: a = a cs - c sn, b = b cs - d sn, c = c cs + a sn,
-// d = d cs + b sn.
-__device__ __forceinline__ void ip_ry_fma(double sn, double cs, double &a, double &b, double &c,
-                                          double &d)
-{
-    double t0, t1, t2, t3;
-    asm volatile(
-        "v_mul_f64 %0, %6, %8\n\t"         // c sn
-        "v_mul_f64 %1, %7, %8\n\t"         // d sn
-        "v_mul_f64 %2, %4, %8\n\t"         // a sn
-        "v_mul_f64 %3, %5, %8\n\t"         // b sn
-        "v_fma_f64 %4, %4, %9, -%0\n\t"
-        "v_fma_f64 %5, %5, %9, -%1\n\t"
-        "v_fma_f64 %6, %6, %9, %2\n\t"
-        "v_fma_f64 %7, %7, %9, %3"
-        : "=&v"(t0), "=&v"(t1), "=&v"(t2), "=&v"(t3), "+v"(a), "+v"(b), "+v"(c), "+v"(d)
-        : "s"(sn), "s"(cs));
-}
-
 // Rotations as three shears (fast sweeps): the real pair (x, y) turned by an
 // angle a with |a| <= pi/2,
 //   x += k1 y;  y += k2 x;  x += k1 y      (k1 = -tan(a/2), k2 = sin a),

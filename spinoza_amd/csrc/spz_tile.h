@@ -44,8 +44,9 @@ enum TileGroup : uint32_t {
     // in-wavefront layout changes: register-index bit `sel` trades places
     // with lane bit 4 (v_permlane16_swap) or lane bit 5 (v_permlane32_swap)
     TG_SW16 = 1u << 18, TG_SW32 = 1u << 19,
-    // fast sweeps (AUTO): an H whose 1/sqrt(2) is applied in place, scaled by
-    // c[0]; plain TG_H then adds and subtracts only (the factor is carried)
+    // fast sweeps (AUTO): an H scaled by c[0] (a controlled H applies its own
+    // 1/sqrt(2)); plain TG_H then adds and subtracts only (the factor of an
+    // uncontrolled H goes to the sweep's global factor)
     TG_HS = 1u << 20,
     // fast sweeps: a general 2x2 complex matrix (c[0..7], row-major, re/im
     // interleaved), for gates conjugated by the sweep's Pauli frame
@@ -103,10 +104,11 @@ static_assert(sizeof(TileOp) == 128, "TileOp must stay 128 bytes");
 constexpr int kTileMaxK = 14;
 constexpr int kTileMaxR = 32;
 
-// Pauli frame of a fast sweep.  The program leaves phi in registers, and the
-// state to store is psi = w X^x Z^z phi (w = i^k; X^x flips the qubits of x,
-// Z^z negates amplitudes whose index has odd parity on z):
-//   psi[j ^ x] = w (-1)^popcount(j & z) phi[j].
+// Pauli frame and global factor of a fast sweep.  The program leaves phi in
+// registers, and the state to store is psi = w X^x Z^z (g phi) (w = i^k; X^x
+// flips the qubits of x, Z^z negates amplitudes whose index has odd parity on
+// z; g = gre + i gim when TF_PHASE is set, else 1):
+//   psi[j ^ x] = w (-1)^popcount(j & z) g phi[j].
 // The host splits x and z by where the store layout holds each bit: x on
 // register bits is folded into gs[], x on lane bits is fxl; z on lane bits is
 // fzl, on register bits the per-register parity mask fzr (bit p), outside the
@@ -130,8 +132,9 @@ struct TileDesc {
                               // and the 1/sqrt(2) of its plain H)
 };
 
-// fast: the AUTO arithmetic (FMA-form RX/RY, H with the factor carried; see
-// the kernel); otherwise every gate is the reference's operation sequence.
+// fast: the AUTO arithmetic (rotations and phases as shears, H and the |0>
+// phases of diagonals carried to the store, the Pauli frame; see the kernel);
+// otherwise every gate is the reference's operation sequence.
 int launch_tile(spz_state_s *s, int K, const TileDesc &d, const TileOp *d_ops, bool fast);
 
 }  // namespace spz

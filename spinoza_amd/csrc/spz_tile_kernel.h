@@ -23,7 +23,9 @@
 // pair_update/diag_update (the reference's per-gate arithmetic, FMA exactly
 // where it writes mul_add), so a sweep that keeps program order is
 // bit-identical to one launch per gate.  FAST (AUTO sweeps) trades that for
-// fewer instructions in H, RX and RY.  Layout changes move data only (LDS
+// fewer instructions: H without its scale, rotations and phases as shears,
+// Paulis and global phases applied once at the store (the host plans them).
+// Layout changes move data only (LDS
 // transposes, or in-wavefront lane swaps) and never change a value.
 #include <type_traits>
 
