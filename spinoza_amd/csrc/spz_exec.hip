@@ -788,7 +788,10 @@ struct Planner {
             }
         };
         reg_offsets(RS, sw.desc.gr);
-        for (int t = 0; t < LOGW; t++) sw.desc.tin[t] = (int8_t)THR[t];
+        for (int t = 0; t < LOGW; t++) {
+            sw.desc.tin[t] = (int8_t)THR[t];
+            sw.desc.tg[t] = 1ull << sw.desc.spos[THR[t]];
+        }
         // fast sweeps: uncontrolled H ops leave out their 1/sqrt(2) and
         // uncontrolled diagonals their phase on |0>; both go into one global
         // factor (gre, gim) that the store applies

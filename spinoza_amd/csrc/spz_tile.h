@@ -130,6 +130,8 @@ struct TileDesc {
     double gre, gim;          // fast sweeps, TF_PHASE: a global factor applied before
                               // the frame (the phases the sweep's diagonals left out
                               // and the 1/sqrt(2) of its plain H)
+    uint64_t tg[16];          // global bit of thread-index bit i in the load layout
+                              // (1 << spos[tin[i]]: uniform, no per-lane table reads)
 };
 
 // fast: the AUTO arithmetic (rotations and phases as shears, H and the |0>

@@ -333,12 +333,13 @@ k_tile(T *__restrict__ re, T *__restrict__ im, const TileDesc d, const TileOp *_
     // thread part of the global index and of the local index (lane part)
     uint64_t gt = 0;
     uint32_t lt = 0;
+    // (branch-free: the uniform table reads are issued together)
 #pragma unroll
-    for (int i = 0; i < LOGW; i++)
-        if ((tid >> i) & 1u) {
-            gt |= 1ull << d.spos[d.tin[i]];
-            lt |= 1u << d.tin[i];
-        }
+    for (int i = 0; i < LOGW; i++) {
+        const uint32_t on = (tid >> i) & 1u;
+        gt |= d.tg[i] & (0ull - (uint64_t)on);
+        lt |= on << d.tin[i];
+    }
 
     double vr[R], vi[R];
 #pragma unroll
