@@ -26,7 +26,7 @@
 
 namespace spz {
 
-struct BlockDesc {
+struct alignas(8) BlockDesc {
     int8_t spos[16];   // tile qubits (ascending), 12 entries
     int8_t bpos[8];    // local positions of the 5 block bits (amplitude index bit a_j)
     int8_t vpos[8];    // local positions of the 7 vector-index bits

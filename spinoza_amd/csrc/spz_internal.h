@@ -223,11 +223,14 @@ __device__ __forceinline__ void diag_update(int ak, const double *k, int bit, do
     }
 }
 
-// Insert zero bits at the ascending positions pos[0..np).
+// Insert zero bits at the ascending positions pos[0..np).  The positions are
+// read as whole words at a uniform index (scalar loads), never as per-lane
+// byte loads, so an index costs no memory round trip per thread.
 __device__ __forceinline__ uint64_t insert_zeros(uint64_t p, const int8_t *pos, int np)
 {
     for (int i = 0; i < np; i++) {
-        int q = pos[i];
+        const uint32_t w = reinterpret_cast<const uint32_t *>(pos)[i >> 2];
+        const int q = (int)(int8_t)(w >> (8 * (i & 3)));
         uint64_t lo = p & ((1ull << q) - 1ull);
         p = ((p >> q) << (q + 1)) | lo;
     }
