@@ -875,75 +875,75 @@ struct Planner {
                 if (sign == 2) zafter = true; // P ran as Z P: Z on phi (after it)
                 split_ops += reps - 1;
             }
-            for (int rep = 0; rep < reps; rep++) {
-            TileOp op;
-            memset(&op, 0, sizeof op);
-            op.ak = g.k.ak;
-            for (int j = 0; j < 8; j++) op.c[j] = g.k.c[j];
-            const bool uncontrolled = !g.ones && !g.zeros;
-            auto slot_of = [&](int b) {
-                for (int j = 0; j < RB; j++)
-                    if (RS[j] == b) return j;
-                return -1;
-            };
-            uint32_t mro = 0, mrz = 0;  // controls on register-index bits (read 1 / 0)
-            for (int q = 0; q < n; q++) {
-                const bool one = (g.ones >> q) & 1, zero = (g.zeros >> q) & 1;
-                if (!one && !zero) continue;
-                const int b = lbit[q];
-                if (b < 0) {
-                    if (one) op.eo |= 1ull << q;
-                    op.em |= 1ull << q;
-                    continue;
+            for (int rep = 0; rep < reps; rep++) {  // (a split gate: two equal halves)
+                TileOp op;
+                memset(&op, 0, sizeof op);
+                op.ak = g.k.ak;
+                for (int j = 0; j < 8; j++) op.c[j] = g.k.c[j];
+                const bool uncontrolled = !g.ones && !g.zeros;
+                auto slot_of = [&](int b) {
+                    for (int j = 0; j < RB; j++)
+                        if (RS[j] == b) return j;
+                    return -1;
+                };
+                uint32_t mro = 0, mrz = 0;  // controls on register-index bits (read 1 / 0)
+                for (int q = 0; q < n; q++) {
+                    const bool one = (g.ones >> q) & 1, zero = (g.zeros >> q) & 1;
+                    if (!one && !zero) continue;
+                    const int b = lbit[q];
+                    if (b < 0) {
+                        if (one) op.eo |= 1ull << q;
+                        op.em |= 1ull << q;
+                        continue;
+                    }
+                    const int j = slot_of(b);
+                    if (j >= 0) {
+                        if (one) mro |= 1u << j; else mrz |= 1u << j;
+                    } else {
+                        if (one) op.mlo |= 1u << b; else op.mlz |= 1u << b;
+                    }
                 }
-                const int j = slot_of(b);
-                if (j >= 0) {
-                    if (one) mro |= 1u << j; else mrz |= 1u << j;
-                } else {
-                    if (one) op.mlo |= 1u << b; else op.mlz |= 1u << b;
-                }
-            }
-            for (uint32_t p = 0; p < (1u << RB); p++)
-                if ((p & mro) == mro && !(p & mrz)) op.pm |= 1u << p;
-            const bool pred = (mro | mrz | op.mlo | op.mlz) != 0;
-            const uint32_t kbit = kind_bit(op.ak);
-            if (!kbit) {
-                bad = true;
-                return;
-            }
-            op.grp = TG_GATE | kbit;
-            if (fast && g.k.ak == AK_H) {
-                if (uncontrolled) hdebt++;
-                else op.grp = (op.grp & ~TG_H) | TG_HS;  // applies its own factor
-            }
-            if (g.diag) {
-                op.type = TOP_DIAG;
-                const int b = lbit[g.target];
-                if (b < 0) {
-                    op.dsel = 2;
-                    op.ebit = 1ull << g.target;
-                    op.grp |= pred ? TG_DLANEP : TG_DLANE;
-                } else if (slot_of(b) >= 0) {
-                    op.dsel = 0;
-                    op.dpos = slot_of(b);
-                    op.sel = 1u << op.dpos;
-                    op.grp |= pred ? TG_DREGP : TG_DREG;
-                } else {
-                    op.dsel = 1;
-                    op.dpos = b;
-                    op.grp |= pred ? TG_DLANEP : TG_DLANE;
-                }
-            } else {
-                op.type = TOP_PAIR;
-                const int slot = slot_of(lbit[g.target]);
-                if (slot < 0) {  // sequence() switches layouts before such a gate
+                for (uint32_t p = 0; p < (1u << RB); p++)
+                    if ((p & mro) == mro && !(p & mrz)) op.pm |= 1u << p;
+                const bool pred = (mro | mrz | op.mlo | op.mlz) != 0;
+                const uint32_t kbit = kind_bit(op.ak);
+                if (!kbit) {
                     bad = true;
                     return;
                 }
-                op.sel = 1u << slot;
-                op.grp |= pred ? TG_PAIRP : TG_PAIR;
-            }
-            prog.push_back(op);
+                op.grp = TG_GATE | kbit;
+                if (fast && g.k.ak == AK_H) {
+                    if (uncontrolled) hdebt++;
+                    else op.grp = (op.grp & ~TG_H) | TG_HS;  // applies its own factor
+                }
+                if (g.diag) {
+                    op.type = TOP_DIAG;
+                    const int b = lbit[g.target];
+                    if (b < 0) {
+                        op.dsel = 2;
+                        op.ebit = 1ull << g.target;
+                        op.grp |= pred ? TG_DLANEP : TG_DLANE;
+                    } else if (slot_of(b) >= 0) {
+                        op.dsel = 0;
+                        op.dpos = slot_of(b);
+                        op.sel = 1u << op.dpos;
+                        op.grp |= pred ? TG_DREGP : TG_DREG;
+                    } else {
+                        op.dsel = 1;
+                        op.dpos = b;
+                        op.grp |= pred ? TG_DLANEP : TG_DLANE;
+                    }
+                } else {
+                    op.type = TOP_PAIR;
+                    const int slot = slot_of(lbit[g.target]);
+                    if (slot < 0) {  // sequence() switches layouts before such a gate
+                        bad = true;
+                        return;
+                    }
+                    op.sel = 1u << slot;
+                    op.grp |= pred ? TG_PAIRP : TG_PAIR;
+                }
+                prog.push_back(op);
             }
             if (zafter) fr.z ^= tb;  // (on phi: the frame takes it on its right)
             if (push == PUSH_H) {  // H X^a Z^b = (-1)^(ab) X^b Z^a H
