@@ -204,6 +204,16 @@ __device__ __forceinline__ double ising_row(const double *pr, const double *pi, 
     return a * vr + b * vi;
 }
 
+// Two ops whose pairs are the same amplitudes, split the same way over the
+// members: equal fixed positions (target and controls) and control values.
+__device__ __forceinline__ bool same_pairs(const BOp &a, const BOp &b)
+{
+    if (a.np != b.np || a.ones != b.ones) return false;
+    for (int i = 0; i < a.np; i++)
+        if (a.pos[i] != b.pos[i]) return false;
+    return true;
+}
+
 // A whole circuit in one workgroup (psi, lambda in the workgroup's slot).
 __device__ void run_single(const BCirc &c, uint64_t ci, const BOp *__restrict__ ops, double hx,
                            double *pr, double *pi, double *lr, double *li, double *S, double *C,
@@ -281,7 +291,9 @@ __device__ void run_single(const BCirc &c, uint64_t ci, const BOp *__restrict__ 
 // barrier around non-local gates (before one, because it reads other blocks,
 // and after it), before lambda (which reads every block), after lambda (the
 // reverse sweep overwrites what other members read) and at the end; runs of
-// local gates need none.  Partial sums go to `part` (one (S, C) pair per
+// local gates need none, and neither do two non-local gates in a row on the
+// same fixed positions and control values (RY then RZ on one qubit): a member
+// then reads and writes the same pairs in both, which no other member touches.  Partial sums go to `part` (one (S, C) pair per
 // member for the energy and for every op) and member 0 adds them, in member
 // order, after the last barrier.  Returns false when a barrier gave up.
 __device__ bool run_group(const BCirc &c, uint64_t ci, unsigned m, unsigned g,
@@ -310,7 +322,8 @@ __device__ bool run_group(const BCirc &c, uint64_t ci, unsigned m, unsigned g,
     for (int o = 0; o < c.nops; o++) {
         const BOp &op = ops[c.op0 + o];
         const bool nl = op.pos[op.np - 1] >= lb;
-        if ((nl || prev_nl) && !sync()) return false;
+        const bool same = o > 0 && nl && prev_nl && same_pairs(op, ops[c.op0 + o - 1]);
+        if ((nl || prev_nl) && !same && !sync()) return false;
         prev_nl = nl;
         const uint64_t tb = 1ull << op.target;
         const uint64_t np = N >> op.np, ps = np / g, plo = ps * m;
@@ -344,7 +357,8 @@ __device__ bool run_group(const BCirc &c, uint64_t ci, unsigned m, unsigned g,
     for (int o = c.nops - 1; o >= 0; o--) {
         const BOp &op = ops[c.op0 + o];
         const bool nl = op.pos[op.np - 1] >= lb;
-        if ((nl || prev_nl) && !sync()) return false;
+        const bool same = o + 1 < c.nops && nl && prev_nl && same_pairs(op, ops[c.op0 + o + 1]);
+        if ((nl || prev_nl) && !same && !sync()) return false;
         prev_nl = nl;
         const uint64_t tb = 1ull << op.target;
         const uint64_t np = N >> op.np, ps = np / g, plo = ps * m;
