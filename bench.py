@@ -82,6 +82,9 @@ def parse():
                         "(0: the library default, < 0: never)")
     p.add_argument("--batch-wg-max", type=int, default=0,
                    help="persistent-kernel switch point (0: autotuned)")
+    p.add_argument("--batch-wg-per-cu", type=int, default=0,
+                   help="resident persistent-kernel workgroups per CU (0: the library "
+                        "default, 2; < 0: as many as fit)")
     p.add_argument("--no-qft", action="store_true", help="skip the 20q QFT leg")
     return p.parse_args()
 
@@ -244,7 +247,7 @@ def fusion_leg(spz, N, layers, n64, n32, n_prec, dense_layers, seed=3):
     return out
 
 
-def batch_leg(spz, N, count, seed=5, group_qubits=0, wg_max=0):
+def batch_leg(spz, N, count, seed=5, group_qubits=0, wg_max=0, wg_per_cu=0):
     """`count` hardware-efficient-ansatz circuits, n ~ U{10..20}, depth ~
     U{2..12} (SplitMix64 seed 5), Ising energy (hx = 1): one batched call
     after autotuning the per-workgroup switch point, against looping the
@@ -263,7 +266,8 @@ def batch_leg(spz, N, count, seed=5, group_qubits=0, wg_max=0):
         circuits.append((n, ops))
         nparams += len(where)
     wg = wg_max or batch_autotune(circuits)
-    run = lambda: batch_adjoint(circuits, 1.0, wg_max_qubits=wg, group_qubits=group_qubits)
+    run = lambda: batch_adjoint(circuits, 1.0, wg_max_qubits=wg, group_qubits=group_qubits,
+                                wg_per_cu=wg_per_cu)
     run()  # warm (graph capture, buffers)
     dt, (energies, grads, st) = timed(run, sync)
     states = {}
@@ -285,6 +289,7 @@ def batch_leg(spz, N, count, seed=5, group_qubits=0, wg_max=0):
         err = max(err, abs(e - e1), float(np.max(np.abs(g - g1))))
     del states
     return {"circuits": count, "params": nparams, "wg_max_qubits": wg,
+            "wg_per_cu": wg_per_cu if wg_per_cu else 2,  # (0: the library default)
             "n_small": st["n_small"], "n_large": st["n_large"], "n_group": st["n_group"],
             "total_ms": round(dt * 1e3, 2), "p50_ms": round(st["p50_ms"], 4),
             "p99_ms": round(st["p99_ms"], 4), "grads_per_s": round(nparams / dt, 1),
@@ -467,7 +472,7 @@ def main():
     if not a.no_batch:
         log(f"batched adjoint leg: {a.batch_count} circuits")
         result["batch"] = batch_leg(spz, N, a.batch_count, group_qubits=a.batch_group_qubits,
-                                    wg_max=a.batch_wg_max)
+                                    wg_max=a.batch_wg_max, wg_per_cu=a.batch_wg_per_cu)
 
     qft_ctx = None
     if not a.no_qft:

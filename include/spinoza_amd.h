@@ -277,7 +277,11 @@ typedef struct spz_batch_opts {
                                qubits run on 2^(n - group_qubits) workgroups
                                of one XCD; 0 = 12, < 0 = never (values
                                below 8 count as 8)                          */
-    int32_t reserved[5];
+    int32_t wg_per_cu;      /* resident persistent-kernel workgroups per CU:
+                               0 = 2 (a group's members then share their CU
+                               with at most one other circuit, which bounds
+                               the per-circuit tail), < 0 = as many as fit  */
+    int32_t reserved[4];
 } spz_batch_opts;
 
 typedef struct spz_batch_stats {

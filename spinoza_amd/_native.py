@@ -66,7 +66,8 @@ class BatchCircuit_t(ctypes.Structure):
 
 class BatchOpts_t(ctypes.Structure):
     _fields_ = [("wg_max_qubits", ctypes.c_int32), ("use_graph", ctypes.c_int32),
-                ("group_qubits", ctypes.c_int32), ("reserved", ctypes.c_int32 * 5)]
+                ("group_qubits", ctypes.c_int32), ("wg_per_cu", ctypes.c_int32),
+                ("reserved", ctypes.c_int32 * 4)]
 
 
 class PlanInfo_t(ctypes.Structure):

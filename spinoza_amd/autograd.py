@@ -115,13 +115,15 @@ class IsingEnergy:
         return cls._fn.apply(thetas, builder, n, hx, ckpt_every, precision)
 
 
-def batch_adjoint(circuits, hx=1.0, wg_max_qubits=0, use_graph=True, group_qubits=0):
+def batch_adjoint(circuits, hx=1.0, wg_max_qubits=0, use_graph=True, group_qubits=0,
+                  wg_per_cu=0):
     """Energies and adjoint gradients of many independent circuits.
 
     `circuits` is a list of (n, ops); every circuit starts in |0...0>.
     Circuits up to `wg_max_qubits` run in the persistent kernel, those above
     `group_qubits` of them on a group of workgroups (0: the defaults; a
-    negative group_qubits keeps every circuit in one workgroup).
+    negative group_qubits keeps every circuit in one workgroup), with
+    `wg_per_cu` resident workgroups per CU (0: 2; negative: as many as fit).
     Returns (energies, [grad_per_op arrays], stats dict)."""
     all_ops, table, off, goff = [], [], 0, 0
     for n, ops in circuits:
@@ -139,6 +141,7 @@ def batch_adjoint(circuits, hx=1.0, wg_max_qubits=0, use_graph=True, group_qubit
     opts.wg_max_qubits = int(wg_max_qubits)
     opts.use_graph = int(bool(use_graph))
     opts.group_qubits = int(group_qubits)
+    opts.wg_per_cu = int(wg_per_cu)
     st = N.BatchStats_t()
     N.check(N.lib().spz_batch_adjoint(cb, len(table), arr, float(hx), N.dptr(energies),
                                       N.dptr(grads), ctypes.byref(opts), ctypes.byref(st)))

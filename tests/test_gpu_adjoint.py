@@ -200,9 +200,11 @@ def test_batched_adjoint_config_vs_oracle(spz, orc):
     energy within 1e-12 and two seeded gradient entries per circuit within
     1e-10 of oracle parameter shift.  wg_max_qubits = 18 puts 10..18 qubits in
     the persistent kernel (10..12 in one workgroup each, 13..18 on groups of
-    2..64 workgroups of one XCD) and 19..20 on the single-circuit path; the
-    call runs with and without the graph, and once more with every circuit of
-    the persistent kernel in one workgroup."""
+    2..64 workgroups of one XCD: with the default two resident workgroups per
+    CU a 64-member group fills its XCD) and 19..20 on the single-circuit path;
+    the call runs with and without the graph, with as many resident workgroups
+    per CU as fit, and once more with every circuit of the persistent kernel in
+    one workgroup."""
     print("case batched adjoint vs oracle", flush=True)
     from spinoza_amd.autograd import batch_adjoint
     from spinoza_amd.rng import SplitMix64
@@ -217,10 +219,11 @@ def test_batched_adjoint_config_vs_oracle(spz, orc):
             circuits.append((n, ops))
             wheres.append(where)
     results = [batch_adjoint(circuits, 1.0, wg_max_qubits=18, use_graph=g) for g in (True, False)]
+    results.append(batch_adjoint(circuits, 1.0, wg_max_qubits=18, wg_per_cu=-1))
     for energies, grads, stats in results:
         assert stats["n_small"] == 18 and stats["n_large"] == 4
         assert stats["n_group"] == 12, stats
-    results.append(batch_adjoint(circuits, 1.0, wg_max_qubits=18, group_qubits=-1))
+    results.append(batch_adjoint(circuits, 1.0, wg_max_qubits=18, group_qubits=-1, wg_per_cu=-1))
     assert results[-1][2]["n_group"] == 0
     for c, ((n, ops), where) in enumerate(zip(circuits, wheres)):
         picks = sorted({where[rng.next_u64() % len(where)] for _ in range(2)})
