@@ -157,6 +157,7 @@ uint32_t kind_bit(int ak)
     case AK_PHASE: return TG_PHASE;
     case AK_RZ: return TG_RZ;
     case AK_MAT: return TG_MAT;
+    case AK_USH: return TG_USH;
     default: return 0;
     }
 }
@@ -305,6 +306,73 @@ int shear_form(Coef &k, bool unc, int *sign)
     memset(k.c, 0, sizeof k.c);
     shear_of(c, s, k.c);
     return reps;
+}
+
+// An uncontrolled 2x2 unitary M (U, or a gate the frame turned into a general
+// matrix) as e^(i al) P(be) RY(2a) P(de), P(x) = diag(1, e^(ix)): |m00| = cos a,
+// |m10| = sin a, a in [0, pi/2], and the phases from whichever entries are
+// the larger (so a near-zero sin or cos does not amplify rounding).  For the
+// shears both phases must have |angle| <= pi/2: P(de) = Z P(de - pi) and
+// Z RY(2a) Z = RY(-2a) turn (be, a, de) into (be + pi, -a, de - pi), and a
+// remaining P(be) becomes Z P(be - pi), the Z going on phi after the op
+// (*zafter).  k gets AK_USH with (-tan(x/2), sin x) for de, a and be; the
+// global phase e^(i al) is returned in (*gr, *gi).
+void ush_form(Coef &k, double *gr, double *gi, bool *zafter)
+{
+    double m[8];
+    gate_matrix2(k, m);
+    const double ca = std::hypot(m[0], m[1]), sa = std::hypot(m[4], m[5]);
+    double ar, ai, br, bi, dr, di;  // e^(i al), e^(i (al + be)), e^(i (al + de))
+    if (ca >= sa) {
+        ar = m[0] / ca;
+        ai = m[1] / ca;
+        const double tr = m[6] / ca, ti = m[7] / ca;  // e^(i (al + be + de))
+        if (sa > 0.0) {
+            br = m[4] / sa;
+            bi = m[5] / sa;
+        } else {
+            br = ar;
+            bi = ai;
+        }
+        const double xr = tr * br + ti * bi, xi = ti * br - tr * bi;  // e^(i de)
+        dr = xr * ar - xi * ai;
+        di = xr * ai + xi * ar;
+    } else {
+        br = m[4] / sa;
+        bi = m[5] / sa;
+        dr = -m[2] / sa;
+        di = -m[3] / sa;
+        if (ca > 0.0) {
+            ar = m[0] / ca;
+            ai = m[1] / ca;
+        } else {
+            ar = 1.0;
+            ai = 0.0;
+        }
+    }
+    // e^(i be), e^(i de): the conjugate of e^(i al) taken out
+    double ebr = br * ar + bi * ai, ebi = bi * ar - br * ai;
+    double edr = dr * ar + di * ai, edi = di * ar - dr * ai;
+    double s = sa;
+    if (edr < 0.0) {
+        edr = -edr;
+        edi = -edi;
+        ebr = -ebr;
+        ebi = -ebi;
+        s = -s;
+    }
+    *zafter = ebr < 0.0;
+    if (*zafter) {
+        ebr = -ebr;
+        ebi = -ebi;
+    }
+    memset(k.c, 0, sizeof k.c);
+    k.ak = AK_USH;
+    shear_of(edr, edi, k.c);
+    shear_of(ca, s, k.c + 2);
+    shear_of(ebr, ebi, k.c + 4);
+    *gr = ar;
+    *gi = ai;
 }
 
 struct Planner {
@@ -869,6 +937,13 @@ struct Planner {
                     g.k.c[0] = d1r * d0r + d1i * d0i;
                     g.k.c[1] = d1i * d0r - d1r * d0i;
                 }
+                if (unc && (g.k.ak == AK_U || g.k.ak == AK_MAT)) {
+                    double pr, pi;
+                    ush_form(g.k, &pr, &pi, &zafter);
+                    const double t = gre * pr - gim * pi;
+                    gim = gre * pi + gim * pr;
+                    gre = t;
+                }
                 int sign = 0;
                 reps = shear_form(g.k, unc, &sign);
                 if (sign == 1) fr.w += 2;     // the op ran as -G: a global -1
@@ -1121,6 +1196,37 @@ int run_fused(spz_state_s *s, const std::vector<SG> &run, int K, bool strict)
     pl.swaps = env_int("SPZ_TILE_SWAPS", 1) != 0;
     pl.plan(run);
     if (pl.bad) return fail(SPZ_ERR_ARG, "tile planner produced a gate outside the register layout");
+    if (pl.fast) {
+        // The fast sweeps' global factors are scalars, and every gate in
+        // between is linear: a sweep may leave its factor to a later one.
+        // Only the last tile sweep of the run applies the product (or an
+        // earlier one, once its magnitude leaves [2^-32, 2^32], so that the
+        // stored amplitudes stay within 2^32 of the state's).
+        size_t last = pl.sweeps.size();
+        for (size_t i = 0; i < pl.sweeps.size(); i++)
+            if (pl.sweeps[i].tile) last = i;
+        double ar = 1.0, ai = 0.0;
+        for (size_t i = 0; i < pl.sweeps.size(); i++) {
+            if (!pl.sweeps[i].tile) continue;
+            TileDesc &d = pl.sweeps[i].desc;
+            if (d.fflags & TF_PHASE) {
+                const double t = ar * d.gre - ai * d.gim;
+                ai = ar * d.gim + ai * d.gre;
+                ar = t;
+            }
+            const double m2 = ar * ar + ai * ai;
+            if (i == last || m2 > 0x1p64 || m2 < 0x1p-64) {
+                d.gre = ar;
+                d.gim = ai;
+                if (ar != 1.0 || ai != 0.0) d.fflags |= TF_PHASE;
+                else d.fflags &= ~TF_PHASE;
+                ar = 1.0;
+                ai = 0.0;
+            } else {
+                d.fflags &= ~TF_PHASE;
+            }
+        }
+    }
     std::lock_guard<std::mutex> lk(g_slot_mu);
     ProgSlot *slot = nullptr;
     if (!pl.prog.empty()) SPZ_TRY(upload_program(pl.prog, &slot));

@@ -109,6 +109,8 @@ enum Arith : int {
     AK_NEG = 7,   // diagonal: negate (Z on |1>)           z_proc_chunk
     AK_PHASE = 8, // diagonal: multiply |1> by (c0, c1)    p_proc_chunk
     AK_RZ = 9,    // diagonal: |0>*(c0,c1), |1>*(c2,c3)    rz_apply_strategy1
+    AK_USH = 10,  // fast sweeps: phase on |1>, RY, phase on |1> as shears
+                  // (c0..c5: three (-tan(a/2), sin a) pairs)
 };
 
 struct Coef {

@@ -39,7 +39,11 @@ enum TileGroup : uint32_t {
     TG_DLANE = 1u << 6,   // diagonal, target on a lane bit or outside the tile
     TG_DLANEP = 1u << 7,
     TG_H = 1u << 8, TG_X = 1u << 9, TG_Y = 1u << 10, TG_RX = 1u << 11, TG_RY = 1u << 12,
-    TG_U = 1u << 13, TG_NEG = 1u << 15, TG_PHASE = 1u << 16,
+    TG_U = 1u << 13,
+    // fast sweeps: an uncontrolled 2x2 as P(b) RY(2a) P(d) (AK_USH), its
+    // global phase carried to the sweep's global factor
+    TG_USH = 1u << 14,
+    TG_NEG = 1u << 15, TG_PHASE = 1u << 16,
     TG_RZ = 1u << 17,
     // in-wavefront layout changes: register-index bit `sel` trades places
     // with lane bit 4 (v_permlane16_swap) or lane bit 5 (v_permlane32_swap)
@@ -128,8 +132,10 @@ struct TileDesc {
     uint32_t pad;
     uint64_t fzg;
     double gre, gim;          // fast sweeps, TF_PHASE: a global factor applied before
-                              // the frame (the phases the sweep's diagonals left out
-                              // and the 1/sqrt(2) of its plain H)
+                              // the frame (the phases the diagonals and general 2x2
+                              // ops left out and the 1/sqrt(2) of the plain H, of
+                              // this sweep and of the earlier sweeps of the same
+                              // run that left theirs to it)
     uint64_t tg[16];          // global bit of thread-index bit i in the load layout
                               // (1 << spos[tin[i]]: uniform, no per-lane table reads)
 };

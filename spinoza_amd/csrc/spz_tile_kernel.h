@@ -53,7 +53,8 @@ constexpr int AK_HSCALED = 100;
 
 // FAST selects the arithmetic of AUTO sweeps: RX and RY as two rotations of
 // real pairs, three FMAs each (shears; the host gives (-tan(a/2), sin a)), H
-// as a sum and a difference whose 1/sqrt(2) the host carries to the store.
+// as a sum and a difference whose 1/sqrt(2) the host carries to the store,
+// an uncontrolled general 2x2 as a phase, an RY and a phase (AK_USH).
 // Without FAST every body is the reference's operation sequence.
 template <int AK, bool FAST>
 __device__ __forceinline__ void ip_pair(const double *k, double &a, double &b, double &c, double &d)
@@ -71,7 +72,11 @@ __device__ __forceinline__ void ip_pair(const double *k, double &a, double &b, d
         if constexpr (FAST) ip_ry_shear(k[0], k[1], a, b, c, d);
         else ip_ry(k[0], k[1], a, b, c, d);
     } else if constexpr (AK == AK_MAT) ip_mat(k, a, b, c, d);
-    else ip_u(k, a, b, c, d);
+    else if constexpr (AK == AK_USH) {  // psi1 e^(id), RY, psi1 e^(ib): 12 FMAs
+        ip_rot<false>(k[0], k[1], c, d);
+        ip_ry_shear(k[2], k[3], a, b, c, d);
+        ip_rot<false>(k[4], k[5], c, d);
+    } else ip_u(k, a, b, c, d);
 }
 
 // Lane-bit exchange of one register pair: the halves of x (register bit 0)
@@ -161,12 +166,15 @@ __device__ __forceinline__ void pair_kinds(uint32_t g, uint32_t sel, double *vr,
         if constexpr (FAST)
             if (hit(g & TG_HS)) pair_sel<R, RB, AK_HSCALED, PRED, FAST>(sel, vr, vi, k, pm);
     }
-    if (hit(g & (TG_RX | TG_RY | TG_U | TG_MAT))) {
+    if (hit(g & (TG_RX | TG_RY | TG_U | TG_MAT | TG_USH))) {
         if (hit(g & TG_RX)) pair_sel<R, RB, AK_RX, PRED, FAST>(sel, vr, vi, k, pm);
         if (hit(g & TG_RY)) pair_sel<R, RB, AK_RY, PRED, FAST>(sel, vr, vi, k, pm);
         if (hit(g & TG_U)) pair_sel<R, RB, AK_U, PRED, FAST>(sel, vr, vi, k, pm);
-        if constexpr (FAST)
+        if constexpr (FAST) {
             if (hit(g & TG_MAT)) pair_sel<R, RB, AK_MAT, PRED, FAST>(sel, vr, vi, k, pm);
+            if constexpr (!PRED)  // (uncontrolled only)
+                if (hit(g & TG_USH)) pair_sel<R, RB, AK_USH, PRED, FAST>(sel, vr, vi, k, pm);
+        }
     }
 }
 
