@@ -322,6 +322,23 @@ __device__ __forceinline__ bool lane_pass(uint32_t lt, uint32_t mlo, uint32_t ml
     return (lt & mlo) == mlo && !(lt & mlz);
 }
 
+// Sum of the single-bit 64-bit terms t[i] over the set bits i of x (the
+// bits i < NB): the per-lane part of a global index.  Each bit becomes an
+// all-ones or all-zero word (one signed bit-field extract) that masks the
+// halves of its term into the sum (one and-or each).
+template <int NB>
+__device__ __forceinline__ uint64_t lane_offset(uint32_t x, const uint64_t *t)
+{
+    uint32_t lo = 0, hi = 0;
+#pragma unroll
+    for (int i = 0; i < NB; i++) {
+        const uint32_t m = (uint32_t)((int32_t)(x << (31 - i)) >> 31);
+        lo |= m & (uint32_t)t[i];
+        hi |= m & (uint32_t)(t[i] >> 32);
+    }
+    return ((uint64_t)hi << 32) | lo;
+}
+
 // Register budget: asking for 6 waves per SIMD makes the compiler fit the
 // kernel in ~108 VGPRs without spills (4 resident waves per SIMD instead of 3
 // at its unconstrained 138); the extra wave lets one workgroup's loads and
@@ -339,22 +356,22 @@ k_tile(T *__restrict__ re, T *__restrict__ im, const TileDesc d, const TileOp *_
     // global bits of the qubits outside S for this tile
     const uint64_t base = insert_zeros((uint64_t)blockIdx.x, d.spos, K);
     // thread part of the global index and of the local index (lane part)
-    uint64_t gt = 0;
-    uint32_t lt = 0;
     // (branch-free: the uniform table reads are issued together)
+    uint32_t lt = 0;
 #pragma unroll
-    for (int i = 0; i < LOGW; i++) {
-        const uint32_t on = (tid >> i) & 1u;
-        gt |= d.tg[i] & (0ull - (uint64_t)on);
-        lt |= on << d.tin[i];
-    }
+    for (int i = 0; i < LOGW; i++) lt |= ((tid >> i) & 1u) << d.tin[i];
+    const uint64_t gt = lane_offset<LOGW>(tid, d.tg);
 
+    // base, the register terms gr[p] and the thread part gt have disjoint
+    // bits, so an index is their sum: a uniform pointer per register (scalar
+    // adds) plus the per-lane gt (one vector add per access)
     double vr[R], vi[R];
 #pragma unroll
     for (int p = 0; p < R; p++) {
-        const uint64_t g = base | gt | d.gr[p];
-        vr[p] = (double)re[g];
-        vi[p] = (double)im[g];
+        const T *pr = re + (base + d.gr[p]);
+        const T *pi = im + (base + d.gr[p]);
+        vr[p] = (double)pr[gt];
+        vi[p] = (double)pi[gt];
     }
 
     const uint64_t ope = reinterpret_cast<uint64_t>(ops + d.nops);
@@ -462,30 +479,35 @@ k_tile(T *__restrict__ re, T *__restrict__ im, const TileDesc d, const TileOp *_
             for (int p = 0; p < R; p++) ip_cmul(d.gre, d.gim, vr[p], vi[p]);
         }
     }
-    uint64_t gs = 0;
+    uint64_t gsb[K];
 #pragma unroll
-    for (int b = 0; b < K; b++)
-        if ((lt >> b) & 1u) gs |= 1ull << d.spos[b];
+    for (int b = 0; b < K; b++) gsb[b] = 1ull << d.spos[b];
+    const uint64_t gs = lane_offset<K>(lt, gsb);
     auto store = [&](auto sign, auto odd) {
         uint32_t sl = 0;  // bit 31: odd parity of the lane and tile parts on z
-        if constexpr (decltype(sign)::value)
+        if constexpr (decltype(sign)::value) {
             sl = (uint32_t)(__builtin_popcount(lphi & d.fzl) +
                             __builtin_popcountll(base & d.fzg)) << 31;
+            // (one word per lane: the per-register part below is a uniform
+            // word XORed in, not a shift folded into every register's mask)
+            asm volatile("" : "+v"(sl));
+        }
 #pragma unroll
         for (int p = 0; p < R; p++) {
-            const uint64_t g = base | gs | d.gs[p];
+            T *pr = re + (base + d.gs[p]);
+            T *pi = im + (base + d.gs[p]);
             double a = vr[p], b = vi[p];
             if constexpr (decltype(odd)::value) {
                 a = vi[p];
                 b = vr[p];
             }
             if constexpr (decltype(sign)::value) {
-                const uint32_t m = sl ^ (((d.fzr >> p) & 1u) << 31);
-                a = flip_sign(a, decltype(odd)::value ? m ^ 0x80000000u : m);
-                b = flip_sign(b, m);
+                const uint32_t u = (d.fzr << (31 - p)) & 0x80000000u;
+                a = flip_sign(a, sl ^ (decltype(odd)::value ? u ^ 0x80000000u : u));
+                b = flip_sign(b, sl ^ u);
             }
-            re[g] = (T)a;
-            im[g] = (T)b;
+            pr[gs] = (T)a;
+            pi[gs] = (T)b;
         }
     };
     using F = std::false_type;
