@@ -575,3 +575,55 @@ def test_fast_rotation_edge_angles_vs_oracle(spz, orc, tile_qubits):
     s = to_gpu(spz, base)
     run_ops(s, ops, mode=N.EXEC_AUTO, tile_qubits=tile_qubits)
     assert_match(s, ref, exact=False, what=f"edge angles K={tile_qubits}")
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("tile_qubits", [10, 12])
+def test_fast_general_2x2_vs_oracle(spz, orc, tile_qubits):
+    """AUTO sweeps run an uncontrolled U, and any gate the Pauli frame turns
+    into a general matrix, as a phase on |1>, an RY and a phase on |1>
+    (three-shear rotations) with its global phase in the sweep's factor.
+    The split takes cos and sin of the RY from |u00| and |u10| and its phases
+    from the larger entries, then moves both phase angles into [-pi/2, pi/2]
+    (a Z into the frame when needed).  U at the switch points: theta where
+    sin or cos of theta/2 is 0 or tiny, phi and lambda at 0, +-pi/2, pi,
+    3 pi/2 and near 0, each on a qubit that an X, Y or Z (absorbed into the
+    frame) may precede, between H and CX/CZ that move the frame; controlled
+    U keep the 2x2 product form.  Within 1e-12 of the oracle."""
+    print(f"case fast general 2x2 K={tile_qubits}", flush=True)
+    from spinoza_amd import _native as N
+    from spinoza_amd.executor import plan_stats, run_ops
+    from spinoza_amd.workloads import op
+    n = 14
+    pi = math.pi
+    thetas = [0.0, pi, 2 * pi, 3 * pi, pi / 2, -pi / 2, 1e-9, pi - 1e-9, 0.7]
+    phases = [0.0, pi / 2, -pi / 2, pi, 1.5 * pi, 1e-12, 2.5]
+    rng = np.random.default_rng(41 + tile_qubits)
+    ops = []
+    i = 0
+    for th in thetas:
+        for ph in phases:
+            for la in phases:
+                q = i % n
+                pre = [None, N.X, N.Y, N.Z][(i // n) % 4]
+                if pre is not None:
+                    ops.append(op(pre, q))
+                if i % 11 == 5:
+                    c = (q + 1 + i % (n - 1)) % n
+                    ops.append(op(N.U, q, theta=th, phi=ph, lam=la, ones=1 << c))
+                else:
+                    ops.append(op(N.U, q, theta=th, phi=ph, lam=la))
+                if i % 5 == 0:
+                    ops.append(op(N.H, int(rng.integers(n))))
+                if i % 3 == 0:
+                    c, t = (int(x) for x in rng.choice(n, 2, replace=False))
+                    ops.append(op(N.X if i % 2 else N.Z, t, ones=1 << c))
+                i += 1
+    st = plan_stats(n, ops, mode=N.EXEC_AUTO, tile_qubits=tile_qubits)
+    assert st["frame_gates"] > 0, st
+    base = orc.random_state(53 + tile_qubits, n)
+    ref = base.copy()
+    orc.execute(ref, ops)
+    s = to_gpu(spz, base)
+    run_ops(s, ops, mode=N.EXEC_AUTO, tile_qubits=tile_qubits)
+    assert_match(s, ref, exact=False, what=f"general 2x2 K={tile_qubits}")
