@@ -492,6 +492,81 @@ struct Planner {
         }
     }
 
+    // Passes a greedy plan of `rem` would make: one per sweep, one half per
+    // gate left to the plain path.
+    double tail_cost(const std::vector<SG> &run, std::vector<int> rem)
+    {
+        double cost = 0.0;
+        while (!rem.empty()) {
+            uint64_t sm;
+            const std::vector<int> ord = collect(run, rem, sm);
+            if (ord.size() < 2) {
+                cost += 0.5;
+                rem.erase(rem.begin());
+                continue;
+            }
+            cost += 1.0;
+            rem = minus(rem, ord);
+        }
+        return cost;
+    }
+
+    // rem without the gates of ord (ord: a subsequence of rem).
+    static std::vector<int> minus(const std::vector<int> &rem, const std::vector<int> &ord)
+    {
+        std::vector<int> out;
+        out.reserve(rem.size());
+        size_t k = 0;
+        for (int idx : rem) {
+            if (k < ord.size() && ord[k] == idx) { k++; continue; }
+            out.push_back(idx);
+        }
+        return out;
+    }
+
+    // End of a run: a greedy plan can leave one gate that no sweep takes, a
+    // whole plain pass for one gate.  When about one sweep's worth of gates
+    // remains after this sweep and its plan would end that way, the tile sets
+    // one swap away from this sweep's are tried, and one whose remainder
+    // plans in fewer passes is kept.  Only for states of 2^28 amplitudes and
+    // more, where a pass takes milliseconds and this search overlaps the
+    // sweeps already queued.
+    void endgame(const std::vector<SG> &run, const std::vector<int> &rem, uint64_t &smask,
+                 std::vector<int> &order)
+    {
+        if (strict || improve <= 0 || n < 28 || n <= K) return;
+        if (order.size() >= rem.size() || rem.size() - order.size() > order.size()) return;
+        const double base = tail_cost(run, minus(rem, order));
+        if (base == std::floor(base)) return;  // no plain gate in the tail
+        const uint64_t fixed = (klo >= 64) ? ~0ull : ((1ull << klo) - 1);
+        uint64_t cand = 0;
+        for (int idx : rem) cand |= run[idx].nd;
+        cand &= ~fixed;
+        double best = base;
+        uint64_t bm = smask;
+        std::vector<int> bo;
+        for (int a = 0; a < n; a++) {
+            if (!((smask >> a) & 1) || ((fixed >> a) & 1)) continue;
+            for (int b = 0; b < n; b++) {
+                if (((smask >> b) & 1) || !((cand >> b) & 1)) continue;
+                const uint64_t m2 = (smask & ~(1ull << a)) | (1ull << b);
+                std::vector<int> o2;
+                take_fixed(run, rem, m2, &o2);
+                if (o2.size() < 2) continue;
+                const double c = tail_cost(run, minus(rem, o2));
+                if (c < best) {
+                    best = c;
+                    bm = m2;
+                    bo.swap(o2);
+                }
+            }
+        }
+        if (best < base) {
+            smask = bm;
+            order.swap(bo);
+        }
+    }
+
     void emit_layout(const std::vector<int> &oldRS, const std::vector<int> &RSv,
                      const std::vector<int> &thr)
     {
@@ -1114,6 +1189,7 @@ struct Planner {
         while (!rem.empty()) {
             uint64_t smask;
             std::vector<int> order = collect(run, rem, smask);
+            if (order.size() >= 2) endgame(run, rem, smask, order);
             if (order.size() < 2) {
                 // nothing to fuse: the first remaining gate runs on the plain path
                 Sweep sw;
@@ -1125,14 +1201,7 @@ struct Planner {
                 continue;
             }
             build(run, order, smask);
-            std::vector<int> nrem;
-            nrem.reserve(rem.size());
-            size_t k = 0;
-            for (int idx : rem) {
-                if (k < order.size() && order[k] == idx) { k++; continue; }
-                nrem.push_back(idx);
-            }
-            rem.swap(nrem);
+            rem = minus(rem, order);
         }
     }
 };

@@ -59,6 +59,34 @@ def test_replica_26q_all_layers(spz, replica, mode):
         assert np.array_equal(re, ref.re) and np.array_equal(im, ref.im)
 
 
+@pytest.mark.timeout(300)
+def test_endgame_plan_28q_auto_vs_plain(spz):
+    """At 2^28 amplitudes and more the planner looks ahead at the end of a
+    run so that no gate is left to a plain pass.  The 28-qubit, 60-layer
+    random circuit of seed 2 is one where the greedy plan did leave one; its
+    AUTO result against the PLAIN one (one kernel per gate, bit-identical to
+    the oracle in the other tests; the oracle is too slow at this size):
+    fidelity 1 within 1e-13, norms and <Z0 Z27> within 1e-12."""
+    from spinoza_amd import _native as N
+    from spinoza_amd.executor import plan_stats, run_ops
+    from spinoza_amd.workloads import random_circuit
+    n = 28
+    ops = random_circuit(n, 60, 2)
+    st = plan_stats(n, ops, mode=N.EXEC_AUTO)
+    assert st["plain_gates"] == 0 and st["tile_gates"] == len(ops), st
+    print(f"case endgame 28q ({len(ops)} gates, {st['sweeps']} sweeps)", flush=True)
+    ref = spz.State(n)
+    run_ops(ref, ops, mode=N.EXEC_PLAIN)
+    s = spz.State(n)
+    run_ops(s, ops, mode=N.EXEC_AUTO)
+    nr, ns = ref.norm(), s.norm()
+    fid = abs(spz.inner(s, ref)) ** 2 / (nr * ns)
+    print(f"endgame 28q: infidelity {1 - fid:.3e}, norms {nr!r} {ns!r}", flush=True)
+    assert abs(1.0 - fid) <= 1e-13
+    assert abs(nr - ns) <= 1e-12
+    assert abs(spz.expect_zz(s, 0, n - 1) - spz.expect_zz(ref, 0, n - 1)) <= 1e-12
+
+
 def shifted_energies(orc, n, ops, idxs, shift, hx):
     """{i: (E(theta_i + shift), E(theta_i - shift))} for the ops idxs."""
     st = orc.OracleState(n)

@@ -48,9 +48,23 @@ def test_tile_set_search_never_worse():
 
 def test_bench_circuit_sweep_count():
     """The 30-qubit, 40-layer bench circuit (1800 gates) fuses into at most 32
-    sweeps of 12-qubit tiles (49 with the first-come tile set)."""
+    sweeps of 12-qubit tiles (49 with the first-come tile set), and the
+    end-of-run lookahead leaves none of its gates to a plain pass (the greedy
+    tile sets left its last gate to one)."""
     st = plan_stats(30, random_circuit(30, 40, 2), tile_qubits=12)
     assert st["sweeps"] <= 32, st
+    assert st["plain_gates"] == 0, st
+
+
+@pytest.mark.parametrize("n,seed", [(28, 2), (30, 3), (32, 5)])
+def test_endgame_lookahead_costs_no_sweeps(n, seed):
+    """Random circuits whose greedy plan ends in a plain gate (28 q seed 2,
+    32 q seed 5) or not (30 q seed 3): every gate still runs exactly once and
+    the sweep counts stay those of the greedy plan (44, 32, 53)."""
+    st = plan_stats(n, random_circuit(n, 40 if n == 30 else 60, seed), tile_qubits=12)
+    assert st["tile_gates"] + st["plain_gates"] == (40 if n == 30 else 60) * (n + n // 2), st
+    assert st["plain_gates"] == 0, st
+    assert st["sweeps"] <= {28: 44, 30: 32, 32: 53}[n], st
 
 
 def test_bench_circuit_layout_changes():
