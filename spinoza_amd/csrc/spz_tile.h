@@ -125,7 +125,8 @@ struct TileDesc {
     int8_t tin[16];           // local bit of thread-index bit i in the load layout
     uint64_t gr[kTileMaxR];   // global offset of register index p in the load layout
     uint64_t gs[kTileMaxR];   // ... and in the store layout (its lane part follows the
-                              // program's last layout change)
+                              // program's last layout change); in amplitudes on the
+                              // host, in bytes in the kernel (the launch scales them)
     int32_t nops;
     uint32_t fflags;          // TileFrameFlags (0: no sign or phase at the store)
     uint32_t fxl, fzl, fzr;   // Pauli frame (fast sweeps; zero otherwise)

@@ -363,15 +363,16 @@ k_tile(T *__restrict__ re, T *__restrict__ im, const TileDesc d, const TileOp *_
     const uint64_t gt = lane_offset<LOGW>(tid, d.tg);
 
     // base, the register terms gr[p] and the thread part gt have disjoint
-    // bits, so an index is their sum: a uniform pointer per register (scalar
-    // adds) plus the per-lane gt (one vector add per access)
+    // bits, so an index is their sum: one per-lane pointer for the tile and
+    // thread part, plus the uniform register term (launch_k hands gr[] and
+    // gs[] over as byte offsets): one vector add per access
     double vr[R], vi[R];
+    const char *lr = reinterpret_cast<const char *>(re + (base + gt));
+    const char *li = reinterpret_cast<const char *>(im + (base + gt));
 #pragma unroll
     for (int p = 0; p < R; p++) {
-        const T *pr = re + (base + d.gr[p]);
-        const T *pi = im + (base + d.gr[p]);
-        vr[p] = (double)pr[gt];
-        vi[p] = (double)pi[gt];
+        vr[p] = (double)*reinterpret_cast<const T *>(lr + d.gr[p]);
+        vi[p] = (double)*reinterpret_cast<const T *>(li + d.gr[p]);
     }
 
     const uint64_t ope = reinterpret_cast<uint64_t>(ops + d.nops);
@@ -483,6 +484,8 @@ k_tile(T *__restrict__ re, T *__restrict__ im, const TileDesc d, const TileOp *_
 #pragma unroll
     for (int b = 0; b < K; b++) gsb[b] = 1ull << d.spos[b];
     const uint64_t gs = lane_offset<K>(lt, gsb);
+    char *sr = reinterpret_cast<char *>(re + (base + gs));
+    char *si = reinterpret_cast<char *>(im + (base + gs));
     auto store = [&](auto sign, auto odd) {
         uint32_t sl = 0;  // bit 31: odd parity of the lane and tile parts on z
         if constexpr (decltype(sign)::value) {
@@ -494,8 +497,6 @@ k_tile(T *__restrict__ re, T *__restrict__ im, const TileDesc d, const TileOp *_
         }
 #pragma unroll
         for (int p = 0; p < R; p++) {
-            T *pr = re + (base + d.gs[p]);
-            T *pi = im + (base + d.gs[p]);
             double a = vr[p], b = vi[p];
             if constexpr (decltype(odd)::value) {
                 a = vi[p];
@@ -506,8 +507,8 @@ k_tile(T *__restrict__ re, T *__restrict__ im, const TileDesc d, const TileOp *_
                 a = flip_sign(a, sl ^ (decltype(odd)::value ? u ^ 0x80000000u : u));
                 b = flip_sign(b, sl ^ u);
             }
-            pr[gs] = (T)a;
-            pi[gs] = (T)b;
+            *reinterpret_cast<T *>(sr + d.gs[p]) = (T)a;
+            *reinterpret_cast<T *>(si + d.gs[p]) = (T)b;
         }
     };
     using F = std::false_type;
@@ -528,9 +529,16 @@ int launch_k(spz_state_s *s, const TileDesc &d, const TileOp *d_ops)
     const uint64_t tiles = s->N >> K;
     if (tiles == 0 || tiles > (1ull << 31) - 1) return fail(SPZ_ERR_ARG, "tile grid out of range");
     const double bytes = 2.0 * (double)s->N * (s->prec == SPZ_F64 ? 16.0 : 8.0);
+    // the kernel takes the register terms of its load and store indexes as
+    // byte offsets (one uniform add per access, no scaling)
+    TileDesc db = d;
+    for (int p = 0; p < (1 << (K - LOGW)); p++) {
+        db.gr[p] *= sizeof(T);
+        db.gs[p] *= sizeof(T);
+    }
     LaunchTimer tm(SPZ_K_TILE, bytes);
     hipLaunchKernelGGL((k_tile<T, K, LOGW, FAST>), dim3((unsigned)tiles), dim3(1 << LOGW), 0,
-                       rt().stream, (T *)s->re, (T *)s->im, d, d_ops);
+                       rt().stream, (T *)s->re, (T *)s->im, db, d_ops);
     SPZ_HIP(hipGetLastError());
     return SPZ_OK;
 }
