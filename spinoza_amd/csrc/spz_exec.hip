@@ -140,6 +140,11 @@ struct Sweep {
     int gate;                  // plain: index into the run
     TileDesc desc;
     size_t op_begin, op_count; // range in the program buffer
+    // fast sweeps: the frame a next tile sweep can take over (Z part and
+    // phase; the X part is always applied at the store)
+    bool can_carry;
+    uint64_t carry_z;
+    int carry_w;
 };
 
 // One-hot TileGroup bit of an arithmetic kind (0: not a tile kind).  AK_MAT
@@ -908,6 +913,9 @@ struct Planner {
         memset(&sw.desc, 0, sizeof sw.desc);
         sw.tile = true;
         sw.gate = -1;
+        sw.can_carry = false;
+        sw.carry_z = 0;
+        sw.carry_w = 0;
         int lbit[64];
         for (int q = 0; q < 64; q++) lbit[q] = -1;
         int i = 0;
@@ -941,6 +949,19 @@ struct Planner {
         size_t hdebt = 0;
         double gre = 1.0, gim = 0.0;
         Frame fr;
+        // Consecutive fast sweeps: the previous one stores with only the X
+        // part of its frame applied, and this one starts from the rest (its Z
+        // part and phase), so the sign work at the store is done once, by
+        // the last tile sweep before a plain pass or the end of the run.
+        if (fast && !sweeps.empty() && sweeps.back().tile && sweeps.back().can_carry) {
+            Sweep &pv = sweeps.back();
+            pv.desc.fzl = 0;
+            pv.desc.fzr = 0;
+            pv.desc.fzg = 0;
+            pv.desc.fflags &= ~(TF_SIGN | TF_ODD);
+            fr.z = pv.carry_z;
+            fr.w = pv.carry_w;
+        }
         size_t li_next = 0;
         for (int gi : seq) {
             if (gi < 0) {
@@ -1176,6 +1197,11 @@ struct Planner {
             sw.desc.fzg = fr.z & ~smask;
             if (sw.desc.fzl || sw.desc.fzr || sw.desc.fzg) sw.desc.fflags |= TF_SIGN;
             if (w & 1) sw.desc.fflags |= TF_SIGN | TF_ODD;
+            // what a next sweep takes over if this one stores X^x phi only:
+            // i^w X^x Z^z = i^(w + 2 |x & z|) Z^z X^x
+            sw.can_carry = fast;
+            sw.carry_z = fr.z;
+            sw.carry_w = (fr.w + 2 * __builtin_popcountll(fr.x & fr.z)) & 3;
         }
         sw.op_count = prog.size() - sw.op_begin;
         sw.desc.nops = (int32_t)sw.op_count;
