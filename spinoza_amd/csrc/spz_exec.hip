@@ -385,7 +385,7 @@ struct Planner {
     int improve = 0;  // local-search passes over the tile set (AUTO only)
     bool strict;
     bool fast = false;   // AUTO arithmetic (see launch_tile): H factors carried
-    bool swaps = true;   // in-wavefront layout changes when they reach the set
+    bool swaps = false;  // in-wavefront layout changes when they reach the set
     bool bad = false;
     uint64_t frame_gates = 0;  // gates absorbed into Pauli frames
     uint64_t split_ops = 0;    // second halves of gates run as two half-angle ops
@@ -1288,7 +1288,11 @@ int run_fused(spz_state_s *s, const std::vector<SG> &run, int K, bool strict)
     pl.improve = env_int("SPZ_TILE_SEARCH", 16);
     pl.strict = strict;
     pl.fast = !strict;
-    pl.swaps = env_int("SPZ_TILE_SWAPS", 1) != 0;
+    // In-wavefront lane swaps are off unless asked for: they leave high qubits
+    // on lane bits 4 and 5 at load time (four 128-byte runs per wave instead
+    // of one 512-byte run), which now costs more than the LDS transposes they
+    // save (30 q bench circuit: 7227 against 7190 gates/s without them).
+    pl.swaps = env_int("SPZ_TILE_SWAPS", 0) != 0;
     pl.plan(run);
     if (pl.bad) return fail(SPZ_ERR_ARG, "tile planner produced a gate outside the register layout");
     if (pl.fast) {
@@ -1569,7 +1573,7 @@ int plan_info(int n, int prec, const spz_op *ops, uint64_t n_ops, const spz_exec
         pl.improve = env_int("SPZ_TILE_SEARCH", 16);
         pl.strict = mode == SPZ_EXEC_STRICT;
         pl.fast = !pl.strict;
-        pl.swaps = env_int("SPZ_TILE_SWAPS", 1) != 0;
+        pl.swaps = env_int("SPZ_TILE_SWAPS", 0) != 0;
         pl.plan(run);
         if (pl.bad) return fail(SPZ_ERR_ARG, "tile planner produced a gate outside the register layout");
         out->tile_gates += pl.frame_gates;

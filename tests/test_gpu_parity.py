@@ -449,14 +449,16 @@ def test_back_to_back_executes_without_sync(spz, orc):
 
 @pytest.mark.gpu
 @pytest.mark.parametrize("tile_qubits", [10, 12])
-def test_lane_swaps_and_fast_sweeps_vs_oracle(spz, orc, tile_qubits):
+def test_lane_swaps_and_fast_sweeps_vs_oracle(spz, orc, tile_qubits, monkeypatch):
     """Sweeps whose layout changes include in-wavefront lane swaps (register
-    bit <-> lane bit 4 or 5, v_permlane16/32_swap), and the AUTO arithmetic:
+    bit <-> lane bit 4 or 5, v_permlane16/32_swap; off by default, turned on
+    here with SPZ_TILE_SWAPS=1), and the AUTO arithmetic:
     runs of H with nothing to absorb their factor before the sweep ends, H
     followed by RX / RY / U / RZ that absorb it, controlled H (which applies
     its own factor).  STRICT (swaps, reference arithmetic) is bit-identical to
     the oracle; AUTO within 1e-12.  The plan must contain lane swaps."""
     print(f"case lane swaps / fast sweeps K={tile_qubits}", flush=True)
+    monkeypatch.setenv("SPZ_TILE_SWAPS", "1")
     from spinoza_amd import _native as N
     from spinoza_amd.executor import plan_stats, run_ops
     from spinoza_amd.workloads import op
